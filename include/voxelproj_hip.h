@@ -1,0 +1,67 @@
+/*
+ * voxelproj_hip -- C ABI of the MI355X (gfx950) exact-trapezoid parallel-beam
+ * projector pair.  Plain pointers and sizes; no framework types.
+ *
+ * Geometry: voxel = detector pixel = 1, rotation centre at the volume
+ * centre, detector centre on the rotation axis.  All arrays are float32,
+ * C-contiguous, resident in device memory of the current HIP device.
+ *
+ *   z_order 0:  volume x (SY, SX, H),  sinogram y (A, PX, H)   -- H fastest
+ *   z_order 2:  volume x (H, SY, SX),  sinogram y (H, A, PX)   -- H slowest
+ *
+ * Every entry point ACCUMULATES into its output (out += op(in)), as the
+ * reference kernels do, and is stream-ordered on `stream` (a hipStream_t;
+ * NULL = the null stream).  Calls return 0 on success or a negative status;
+ * vp_last_error() then describes the failure (per calling thread).
+ */
+#ifndef VOXELPROJ_HIP_H
+#define VOXELPROJ_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VP_OK 0
+#define VP_ERR_ARG -1     /* invalid argument (z_order, extents, null pointer) */
+#define VP_ERR_LIMIT -2   /* extents beyond what the kernels support */
+#define VP_ERR_HIP -3     /* HIP runtime error (launch, allocation) */
+
+#if defined(__GNUC__)
+#define VP_API __attribute__((visibility("default")))
+#else
+#define VP_API
+#endif
+
+/*
+ * Forward projection  y += A x.
+ * Replaces the reference binding _CU.forward_z0 / _CU.forward_z2
+ * (voxelproj/csrc/voxelproj.cpp:15-18,25-28) and the host launchers
+ * forward_z0 / forward_z2 (voxelproj/csrc/projectors.cu:34-104, 171-234).
+ * `angles` holds n_angles float32 radians on the device; n_angles is the
+ * sinogram's angle extent.
+ */
+VP_API int vp_forward(int z_order, const float *x, float *y, const float *angles, int n_angles,
+               int64_t SY, int64_t SX, int64_t H, int64_t PX, void *stream);
+
+/*
+ * Backprojection  x += AT y  (exact adjoint of vp_forward).
+ * Replaces _CU.backward_z0 / _CU.backward_z2 (voxelproj/csrc/voxelproj.cpp:20-23,30-33)
+ * and the host launchers backward_z0 / backward_z2
+ * (voxelproj/csrc/projectors.cu:106-169, 236-299).
+ */
+VP_API int vp_backward(int z_order, const float *y, float *x, const float *angles, int n_angles,
+                int64_t SY, int64_t SX, int64_t H, int64_t PX, void *stream);
+
+/* Description of the last failure on the calling thread ("" if none). */
+VP_API const char *vp_last_error(void);
+
+/* ABI version of this header (incremented on incompatible changes). */
+VP_API int vp_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VOXELPROJ_HIP_H */

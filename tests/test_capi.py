@@ -1,0 +1,75 @@
+"""The C-ABI library loads and exports every symbol declared in include/*.h.
+
+Only argument validation is exercised here (it runs on the host before any
+HIP call), so these tests need no GPU.
+"""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+from tests.conftest import ROOT
+
+import voxelproj_amd
+from voxelproj_amd import _lib
+
+
+def _declared_symbols():
+    names = set()
+    for hdr in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(hdr).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"VP_API\s+[\w\s\*]+?\b(vp_\w+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_declares_entry_points():
+    names = _declared_symbols()
+    assert {"vp_forward", "vp_backward", "vp_last_error", "vp_abi_version"} <= names
+
+
+def test_library_exports_declared_symbols():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in _declared_symbols():
+        assert hasattr(lib, name), name
+    assert set(_lib.EXPORTS) <= _declared_symbols()
+
+
+def test_abi_version():
+    assert voxelproj_amd.load().vp_abi_version() == _lib.ABI_VERSION
+
+
+def _call(fn, z=0, ptr=1, A=3, SY=4, SX=4, H=2, PX=5):
+    lib = voxelproj_amd.load()
+    p = ctypes.c_void_p(ptr) if ptr else None
+    return getattr(lib, fn)(z, p, p, p, A, SY, SX, H, PX, None)
+
+
+@pytest.mark.parametrize("fn", ["vp_forward", "vp_backward"])
+def test_bad_z_order_rejected(fn):
+    assert _call(fn, z=1) == -1
+    assert b"z_order" in voxelproj_amd.load().vp_last_error()
+
+
+@pytest.mark.parametrize("fn", ["vp_forward", "vp_backward"])
+def test_null_pointer_rejected(fn):
+    assert _call(fn, ptr=0) == -1
+    assert b"null" in voxelproj_amd.load().vp_last_error()
+
+
+@pytest.mark.parametrize("fn", ["vp_forward", "vp_backward"])
+def test_negative_and_oversized_extents_rejected(fn):
+    assert _call(fn, SX=-1) == -1
+    assert _call(fn, PX=40000) == -2
+    assert _call(fn, A=70000) == -2
+
+
+@pytest.mark.parametrize("fn", ["vp_forward", "vp_backward"])
+def test_empty_problem_is_a_no_op(fn):
+    # zero extents return success without touching the (null) pointers
+    assert _call(fn, ptr=0, A=0) == 0
+    assert _call(fn, ptr=0, H=0) == 0
+    assert voxelproj_amd.load().vp_last_error() == b""

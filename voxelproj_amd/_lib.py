@@ -1,0 +1,71 @@
+"""ctypes loader for libvoxelproj_hip.so (the HIP C-ABI library).
+
+There is no CPU fallback: if the library is missing or cannot be loaded,
+``load()`` raises, and every projector call fails loudly.
+"""
+import ctypes
+import os
+import threading
+
+import torch  # noqa: F401  -- loads torch's HIP runtime first, so the library binds to it
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvoxelproj_hip.so")
+
+EXPORTS = ("vp_forward", "vp_backward", "vp_last_error", "vp_abi_version")
+ABI_VERSION = 1
+
+_lib = None
+_lock = threading.Lock()
+
+
+class VoxelprojLibraryError(RuntimeError):
+    """The HIP projector library is missing, stale or failed to load."""
+
+
+def build(verbose=False):
+    """Compile the library in-tree with hipcc for gfx950."""
+    from .csrc import build as _b
+
+    global _lib
+    path = _b.build(verbose=verbose)
+    return path
+
+
+def load():
+    """Return the loaded library (loading it on first use)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise VoxelprojLibraryError(
+                f"{LIB_PATH} not found: build it with `python voxelproj_amd/csrc/build.py`"
+            )
+        try:
+            lib = ctypes.CDLL(LIB_PATH)
+        except OSError as e:  # pragma: no cover - depends on the host
+            raise VoxelprojLibraryError(f"cannot load {LIB_PATH}: {e}") from e
+        i64 = ctypes.c_int64
+        sig = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+               i64, i64, i64, i64, ctypes.c_void_p]
+        for name in ("vp_forward", "vp_backward"):
+            f = getattr(lib, name)
+            f.argtypes = sig
+            f.restype = ctypes.c_int
+        lib.vp_last_error.argtypes = []
+        lib.vp_last_error.restype = ctypes.c_char_p
+        lib.vp_abi_version.argtypes = []
+        lib.vp_abi_version.restype = ctypes.c_int
+        if lib.vp_abi_version() != ABI_VERSION:
+            raise VoxelprojLibraryError("libvoxelproj_hip.so ABI version mismatch; rebuild it")
+        _lib = lib
+        return lib
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = load().vp_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")

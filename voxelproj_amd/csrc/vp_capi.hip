@@ -1,0 +1,107 @@
+// C ABI of the projector library (declared in include/voxelproj_hip.h).
+// Validation happens here, before anything touches the device; kernels are
+// launched stream-ordered on the caller's stream and every launch is checked.
+#include <stdio.h>
+#include <string>
+
+#include "../../include/voxelproj_hip.h"
+#include "vp_device.hpp"
+#include "vp_launch.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+    return fail(VP_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+constexpr int64_t kMaxExtent = 32768;  // keeps the hi/lo position split accurate
+
+int validate(int z_order, const void* in, const void* out, const float* angles, int n_angles,
+             int64_t SY, int64_t SX, int64_t H, int64_t PX, vp::Geom* g, bool* empty)
+{
+    if (z_order != 0 && z_order != 2)
+        return fail(VP_ERR_ARG, "z_order must be 0 or 2, got " + std::to_string(z_order));
+    if (SY < 0 || SX < 0 || H < 0 || PX < 0 || n_angles < 0)
+        return fail(VP_ERR_ARG, "extents must be non-negative");
+    if (SY > kMaxExtent || SX > kMaxExtent || PX > kMaxExtent)
+        return fail(VP_ERR_LIMIT, "SY, SX and PX must be <= 32768");
+    if (n_angles > 65535)
+        return fail(VP_ERR_LIMIT, "at most 65535 angles per call");
+    if (H > (int64_t)64 * 65535)
+        return fail(VP_ERR_LIMIT, "H must be <= 4194240");
+    *empty = (SY == 0 || SX == 0 || H == 0 || PX == 0 || n_angles == 0);
+    if (!*empty && (in == nullptr || out == nullptr || angles == nullptr))
+        return fail(VP_ERR_ARG, "null device pointer");
+    g->SY = SY;
+    g->SX = SX;
+    g->H = H;
+    g->PX = PX;
+    g->A = n_angles;
+    g->bits = vp::hi_bits(SY, SX, PX);
+    return VP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vp_abi_version(void) { return 1; }
+
+const char* vp_last_error(void) { return g_last_error.c_str(); }
+
+int vp_forward(int z_order, const float* x, float* y, const float* angles, int n_angles,
+               int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
+{
+    g_last_error.clear();
+    vp::Geom g;
+    bool empty = false;
+    int rc = validate(z_order, x, y, angles, n_angles, SY, SX, H, PX, &g, &empty);
+    if (rc != VP_OK || empty) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e;
+    if (z_order == 0) {
+        e = vp::launch_forward_z0(x, y, angles, g, 1, s);
+        if (e != hipSuccess) return hip_fail(e, "forward z_order 0 launch");
+        return VP_OK;
+    }
+    // z_order 2: transpose the volume to slice-fastest order, project, and
+    // add the transposed sinogram back.
+    const int64_t nx = SY * SX * H;
+    const int64_t ny = (int64_t)n_angles * PX * H;
+    float* ws = nullptr;
+    e = hipMallocAsync((void**)&ws, (size_t)(nx + ny) * sizeof(float), s);
+    if (e != hipSuccess) return hip_fail(e, "forward z_order 2 workspace");
+    float* x0 = ws;
+    float* y0 = ws + nx;
+    e = vp::launch_transpose(x, x0, H, SY * SX, 0, s);
+    if (e == hipSuccess) e = vp::launch_forward_z0(x0, y0, angles, g, 0, s);
+    if (e == hipSuccess) e = vp::launch_transpose(y0, y, (int64_t)n_angles * PX, H, 1, s);
+    hipError_t ef = hipFreeAsync(ws, s);
+    if (e != hipSuccess) return hip_fail(e, "forward z_order 2 launch");
+    if (ef != hipSuccess) return hip_fail(ef, "forward z_order 2 workspace release");
+    return VP_OK;
+}
+
+int vp_backward(int z_order, const float* y, float* x, const float* angles, int n_angles,
+                int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
+{
+    g_last_error.clear();
+    vp::Geom g;
+    bool empty = false;
+    int rc = validate(z_order, y, x, angles, n_angles, SY, SX, H, PX, &g, &empty);
+    if (rc != VP_OK || empty) return rc;
+    hipError_t e = vp::launch_backward(z_order, y, x, angles, g, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "backward launch");
+    return VP_OK;
+}
+
+}  // extern "C"

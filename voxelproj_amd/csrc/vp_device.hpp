@@ -1,0 +1,121 @@
+// Device-side geometry and footprint math shared by the projector kernels.
+//
+// Geometry (reference: voxelproj/csrc/include/sp_backward_z0.cuh:24-26,84 and
+// sp_forward_z0.cuh:132-156): voxel size = detector pixel size = 1, rotation
+// centre at the volume centre, detector centre on the rotation axis.  The
+// detector coordinate of voxel (ix, iy) at angle phi is
+//     u = (ix - xc) cos(phi) + (iy - yc) sin(phi) + pc,
+// xc = (SX-1)/2, yc = (SY-1)/2, pc = (PX-1)/2.  The weight of pixel p is the
+// area of the unit voxel inside the unit detector strip around p:
+//     W = F(p + 1/2 - u) - F(p - 1/2 - u),
+// F the CDF of the unit-area trapezoid with plateau half-width
+// a = ||c|-|s||/2, base half-width b = (|c|+|s|)/2 and height h = 1/max(|c|,|s|)
+// (projection_common.cuh:139-166 evaluates the same F).  The support is at
+// most the three pixels r-1, r, r+1 around r = round(u).
+//
+// Numerics.  u reaches several hundred, so forming it in FP32 would put an
+// absolute error of ~1e-5 on every footprint position.  Instead cos and sin
+// are split on the host side of each angle (in FP64) into a "hi" part with
+// B fractional bits and a small "lo" rest.  With B chosen from the volume
+// extents, (ix-xc)*c_hi + (iy-yc)*s_hi + pc is exact in FP32, and only the
+// tiny lo term and the final fractional offset round.  Positions are then
+// good to ~3e-8 and the weights to ~1e-7 absolute.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vp {
+
+// Per-angle constants, rounded once from FP64.
+struct AngleK {
+    float chi, clo;  // cos = chi + clo, chi a multiple of 2^-B
+    float shi, slo;  // sin = shi + slo
+    float a, b;      // plateau / base half-widths of the footprint trapezoid
+    float h;         // plateau height 1/max(|c|,|s|)
+    float cd;        // h / (2 (b - a)); 0 when a == b (quadratic flanks vanish)
+};
+
+__device__ __forceinline__ AngleK make_angle(float phi, double scale, double inv_scale)
+{
+    const double c = cos((double)phi);
+    const double s = sin((double)phi);
+    const double chi = rint(c * scale) * inv_scale;
+    const double shi = rint(s * scale) * inv_scale;
+    const double ac = fabs(c), as = fabs(s);
+    const double a = 0.5 * fabs(ac - as);
+    const double b = 0.5 * (ac + as);
+    const double h = 1.0 / fmax(ac, as);
+    AngleK k;
+    k.chi = (float)chi;
+    k.clo = (float)(c - chi);
+    k.shi = (float)shi;
+    k.slo = (float)(s - shi);
+    k.a = (float)a;
+    k.b = (float)b;
+    k.h = (float)h;
+    const float bf = (float)b, af = (float)a;
+    k.cd = (bf > af) ? 0.5f * k.h / (bf - af) : 0.0f;
+    return k;
+}
+
+// Q(z) = F(-z) for z >= 0 (left tail of the footprint CDF); also valid for
+// z slightly below 0, where the linear plateau piece continues.
+__device__ __forceinline__ float tail(float z, const AngleK& k)
+{
+    const float lin = fmaf(-k.h, z, 0.5f);
+    const float e = fmaxf(k.b - z, 0.0f);
+    const float quad = (k.cd * e) * e;
+    return (z <= k.a) ? lin : quad;
+}
+
+// Footprint of one voxel at one angle: pixel r and weights of r-1, r, r+1.
+struct Taps {
+    int r;
+    float wm, w0, wp;
+};
+
+// r = round(u), computed from (uhi + 1/2) + ulo: uhi + 1/2 is exact and the
+// single rounding of the sum moves r only when u is within ~1e-4 of a
+// half-integer, where either neighbour is a valid centre (|u - r| stays
+// within 1/2 + 1e-4, and the footprint tails are linear there to second
+// order).
+__device__ __forceinline__ float centre_pixel(float uhi, float ulo)
+{
+    return floorf((uhi + 0.5f) + ulo);
+}
+
+// X = ix - xc, Y = iy - yc (exact half-integers); pc = (PX-1)/2.
+__device__ __forceinline__ Taps voxel_taps(float X, float Y, float pc, const AngleK& k)
+{
+    const float uhi = fmaf(X, k.chi, fmaf(Y, k.shi, pc));  // exact
+    const float ulo = fmaf(X, k.clo, Y * k.slo);           // small
+    const float rf = centre_pixel(uhi, ulo);
+    const float t = (uhi - rf) + ulo;                      // u - r; uhi - rf exact
+    Taps o;
+    o.r = (int)rf;
+    o.wm = tail(0.5f + t, k);   // pixel r-1: F(-1/2 - t)
+    o.wp = tail(0.5f - t, k);   // pixel r+1: 1 - F(1/2 - t)
+    o.w0 = (1.0f - o.wm) - o.wp;
+    return o;
+}
+
+// Centre pixel only, for window and segment bounds.
+__device__ __forceinline__ int voxel_r(float X, float Y, float pc, const AngleK& k)
+{
+    const float uhi = fmaf(X, k.chi, fmaf(Y, k.shi, pc));
+    const float ulo = fmaf(X, k.clo, Y * k.slo);
+    return (int)centre_pixel(uhi, ulo);
+}
+
+// Number of fractional bits of the hi split for a volume/detector extent.
+// Every partial sum of (ix-xc)*c_hi + (iy-yc)*s_hi + pc is a multiple of
+// 2^-(B+1) below 2^E in magnitude, so E + B + 1 <= 24 keeps it exact.
+__host__ __device__ inline int hi_bits(int64_t SY, int64_t SX, int64_t PX)
+{
+    const double m = 0.5 * (double)(SX - 1) + 0.5 * (double)(SY - 1) + 0.5 * (double)(PX - 1) + 2.0;
+    int e = 0;
+    while ((double)(1ll << e) < m) ++e;
+    return 23 - e;
+}
+
+}  // namespace vp
