@@ -1,0 +1,228 @@
+"""Benchmark of the exact-trapezoid projector pair on one MI355X.
+
+A step is one forward projection A and one backprojection AT of the
+plain-operator headline workload: float32 volume x ~ N(0,1), shape
+(SY, SX, H) = (512, 512, 512) in z_order 0, 192 angles linspace(0, pi),
+PX = 512 (reference default max(SY, SX)), preallocated outputs, inputs
+resident in HBM.  value = voxel-angles processed per second by the step
+(2 * 512^3 * 192 per step) in Gvoxel*angle/s.
+
+Also reported:
+  roofline      for the dominant kernel, from HIP-event timings on the
+                stream the kernels run on; algorithmic flops = 2*H*nnz(W)
+                (one FMA per nonzero weight per slice), nnz counted exactly
+  per_op        A and AT rates, and both ops in z_order 2
+  cpu_baseline  the OpenMP FP32 transcription of the reference algorithm
+                (oracle/, "port") timed on this host on a bounded sample
+  traffic       HBM bytes per launch when a PMC summary is supplied
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3    # MI355X published FP32 vector peak (packed)
+PEAK_HBM_GBS = 8000.0       # MI355X HBM3E
+
+
+def count_nnz(angles, SY, SX, PX):
+    """Exact number of nonzero weights: pixels p in [0, PX) with |p - u| < b + 1/2."""
+    iy, ix = np.meshgrid(np.arange(SY, dtype=np.float64), np.arange(SX, dtype=np.float64),
+                         indexing="ij")
+    X = (ix - 0.5 * (SX - 1)).ravel()
+    Y = (iy - 0.5 * (SY - 1)).ravel()
+    pc = 0.5 * (PX - 1)
+    total = 0
+    for phi in np.asarray(angles, np.float32).astype(np.float64):
+        c, s = np.cos(phi), np.sin(phi)
+        half = 0.5 * (abs(c) + abs(s)) + 0.5
+        u = X * c + Y * s + pc
+        lo = np.maximum(np.floor(u - half) + 1, 0)
+        hi = np.minimum(np.ceil(u + half) - 1, PX - 1)
+        total += int(np.maximum(hi - lo + 1, 0).sum())
+    return total
+
+
+def time_op(fn, args, n, stream):
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(n):
+        fn(*args)
+    end.record(stream)
+    end.synchronize()
+    return start.elapsed_time(end) / n
+
+
+def cpu_baseline(N, A, H_sample):
+    """Reference algorithm (FP32 transcription, OpenMP) on H_sample slices."""
+    import oracle as vpo
+
+    g = torch.Generator().manual_seed(5)
+    ang = np.linspace(0, np.pi, A).astype(np.float32)
+    x = torch.randn((N, N, H_sample), generator=g).numpy()
+    y = torch.randn((A, N, H_sample), generator=g).numpy()
+    vpo.ref_forward_f32(x[:, :, :1], ang, N, 0)  # warm up threads / pages
+    t0 = time.perf_counter()
+    vpo.ref_forward_f32(x, ang, N, 0)
+    t1 = time.perf_counter()
+    vpo.ref_backward_f32(y, ang, N, N, 0)
+    t2 = time.perf_counter()
+    work = 2.0 * N * N * H_sample * A
+    return {
+        "value": round(work / (t2 - t0) / 1e9, 3),
+        "unit": "Gvoxel*angle/s",
+        "cores": vpo.max_threads(),
+        "kind": "port",
+        "sample": (f"A then AT, FP32 transcription of the reference kernels, "
+                   f"{N}x{N}x{H_sample} slices x {A} angles (z_order 0), "
+                   f"A {t1 - t0:.2f} s, AT {t2 - t1:.2f} s"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--angles", type=int, default=192)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-slices", type=int, default=128)
+    ap.add_argument("--profile", action="store_true",
+                    help="short run for rocprofv3 counters: one 512 MiB copy (reference "
+                         "traffic), then one A and one AT launch; prints no JSON")
+    ap.add_argument("--no-z2", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the projector path is not sharded across GPUs "
+              "(see DESIGN.md for how it would shard along H). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import voxelproj_amd as vp
+    from voxelproj_amd import _CU
+
+    vp.load()
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    N, A = args.size, args.angles
+    PX = N
+    ang_np = np.linspace(0, np.pi, A).astype(np.float32)
+    ang = torch.tensor(ang_np, device=dev)
+    g = torch.Generator(device=dev).manual_seed(0)
+    x = torch.randn((N, N, N), device=dev, generator=g)
+    yb = torch.randn((A, PX, N), device=dev, generator=g)
+    y = torch.zeros((A, PX, N), device=dev)
+    xb = torch.zeros((N, N, N), device=dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def step():
+        _CU.forward_z0(x, y, ang)
+        _CU.backward_z0(yb, xb, ang)
+
+    if args.profile:
+        step()
+        torch.cuda.synchronize()
+        src = torch.empty(128 * 1024 * 1024, device=dev)  # 512 MiB
+        src.fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)                                     # reads 512 MiB, writes 512 MiB
+        step()
+        torch.cuda.synchronize()
+        print("profile run done: copy 512 MiB, A z0, AT z0", flush=True)
+        return
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    vox_ang = float(N) * N * N * A
+    value = 2.0 * vox_ang / (ms_step * 1e-3) / 1e9
+
+    # Per-kernel launch durations (HIP events on the kernels' stream).
+    n_t = max(3, args.steps)
+    ms_a = time_op(_CU.forward_z0, (x, y, ang), n_t, stream)
+    ms_at = time_op(_CU.backward_z0, (yb, xb, ang), n_t, stream)
+    per_op = {"A_z0_ms": round(ms_a, 4), "AT_z0_ms": round(ms_at, 4),
+              "A_z0_gvox_ang_s": round(vox_ang / ms_a / 1e6, 1),
+              "AT_z0_gvox_ang_s": round(vox_ang / ms_at / 1e6, 1)}
+    if not args.no_z2:
+        del y, xb
+        x2 = x.permute(2, 0, 1).contiguous()
+        del x
+        y2 = torch.zeros((N, A, PX), device=dev)
+        yb2 = yb.permute(2, 0, 1).contiguous()
+        del yb
+        xb2 = torch.zeros((N, N, N), device=dev)
+        ms_a2 = time_op(_CU.forward_z2, (x2, y2, ang), n_t, stream)
+        ms_at2 = time_op(_CU.backward_z2, (yb2, xb2, ang), n_t, stream)
+        per_op.update({"A_z2_ms": round(ms_a2, 4), "AT_z2_ms": round(ms_at2, 4),
+                       "A_z2_gvox_ang_s": round(vox_ang / ms_a2 / 1e6, 1),
+                       "AT_z2_gvox_ang_s": round(vox_ang / ms_at2 / 1e6, 1)})
+
+    nnz = count_nnz(ang_np, N, N, PX)
+    flops = 2.0 * N * nnz
+    dom_name, dom_ms = ("forward_z0 (A)", ms_a) if ms_a >= ms_at else ("backproject z0 (AT)", ms_at)
+    achieved = flops / (dom_ms * 1e-3) / 1e12
+    bytes_a = 4.0 * (N ** 3 + 2.0 * A * PX * N)
+    bytes_at = 4.0 * (A * PX * N + 2.0 * N ** 3)
+    dom_bytes = bytes_a if ms_a >= ms_at else bytes_at
+    roofline = {
+        "bound": "valu",
+        "kernel": dom_name,
+        "achieved": round(achieved, 3),
+        "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+        "traffic": None,
+        "flops_per_launch": flops,
+        "nnz_per_voxel_angle": round(nnz / (N * N * A), 5),
+        "hbm_algorithmic_GBs": round(dom_bytes / (dom_ms * 1e-3) / 1e9, 1),
+        "hbm_frac": round(dom_bytes / (dom_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+    }
+    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            tr = json.load(f)
+        roofline["traffic"] = tr.get("forward_z0" if ms_a >= ms_at else "backproject_z0")
+
+    out = {
+        "metric": "A & AT Gvoxel*angle/s at 512^3 x 192 angles (plain operator)",
+        "value": round(value, 2),
+        "unit": "Gvoxel*angle/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded N(0,1) volume and sinogram)",
+        "config": {"workload": f"A then AT, z_order 0, {N}^3 volume, {A} angles linspace(0,pi), "
+                               f"PX={PX}", "size": N, "angles": A, "PX": PX},
+        "roofline": roofline,
+        "per_op": per_op,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(N, A, args.cpu_slices)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
