@@ -7,12 +7,13 @@
 //   wave  = one angle x P consecutive detector pixels x 64 H slices (lane = slice);
 //   rows  = lines of the volume along the axis whose direction cosine is
 //           larger (|kappa| >= 1/sqrt 2), so at most two voxels of a line
-//           share a centre pixel r;
+//           share a centre pixel r and consecutive voxels step r by 0, 1
+//           (or, within rounding near 0 deg, 2);
 //   per line, lane m evaluates the footprint of the m-th voxel of the line
-//   segment that touches the wave's pixels (in increasing-r order), the
-//   segment's slices are staged in LDS, and a statically unrolled sweep over
-//   r = p0-1 .. p0+P adds each voxel to accumulators r-1, r, r+1 -- so every
-//   accumulator index is a compile-time constant.
+//   segment that touches the wave's pixels (in increasing-r order), and a
+//   statically unrolled sweep over r = p0-1 .. p0+P adds each voxel to
+//   accumulators r-1, r, r+1 -- so every accumulator index is a
+//   compile-time constant and only the voxel index is dynamic.
 // No atomics; the summation order is fixed, so results are deterministic.
 #include "vp_device.hpp"
 #include "vp_launch.hpp"
@@ -24,127 +25,335 @@ __device__ __forceinline__ float rdlane(float v, int l)
 {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
 }
-__device__ __forceinline__ int rdlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
-__device__ __forceinline__ int popc64(uint64_t m) { return __builtin_popcountll(m); }
-
-constexpr int kSeg = 64;       // max voxels of a line segment (one per lane)
+constexpr int kSeg = 64;       // max voxels of one wave's line segment (one per lane)
 constexpr int kRowsPerPart = 16;
+constexpr int kXcds = 8;       // MI355X dispatches workgroup i to XCD i % 8
 
-template <int P, int NW>
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uni(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
+// One staged piece of work: voxels [j0, j1] of line `row` (-1 = none).
+struct Task {
+    int row, j0, j1;
+};
+
+// Block = NW angles (one per wave) x P detector pixels x 64*HL slices.
+// For every line of the volume along the angles' dominant axis, the union of
+// the NW line segments that touch the block's pixels is staged in LDS once
+// (HL slices per lane, 256*HL bytes per voxel) and read by all NW waves, so
+// each volume load feeds up to NW angles.  Lines are split into pieces of at
+// most UMAX voxels; the next piece is fetched into registers while the
+// current one is processed (double-buffered LDS, one barrier per piece).
+// Angles whose dominant axis is x and those whose dominant axis is y are
+// handled in two passes over the volume.
+template <int P, int NW, int HL, int UMAX>
 __global__ __launch_bounds__(64 * NW) void forward_z0_kernel(
     const float* __restrict__ x, float* __restrict__ y, const float* __restrict__ angles,
-    int A, int SY, int SX, int H, int PX, double scale, double inv_scale, int accumulate)
+    int A, int SY, int SX, int H, int PX, double scale, double inv_scale, int accumulate,
+    int nbp, int nba)
 {
-    // Segment length bound: the r range [p0-1, p0+P] plus one pixel of margin
-    // on both sides spans P+5 detector units, i.e. at most (P+5)*sqrt 2 + 4
-    // voxels along a line.
     static_assert((P + 5) * 1.4143 + 4 <= kSeg, "segment may exceed one wave");
-    __shared__ float s_x[NW][kSeg][64];
+    constexpr int NT = 64 * NW;
+    constexpr int VOX = 64 * HL;                 // floats per staged voxel
+    constexpr int STAGE = UMAX * VOX;            // floats per buffer
+    constexpr int PER = (STAGE / 4 + NT - 1) / NT;  // float4 per thread per piece
+    __shared__ float4 s_x4[2][STAGE / 4];
+    __shared__ AngleK s_k[NW];
+    __shared__ int s_mode[NW];
+    // Per wave: footprint slots [rank][centre pixel q] = (wm, w0, wp, LDS offset).
+    __shared__ float4 s_t[NW][2][P + 2];
 
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int a = blockIdx.y;
-    const int h = blockIdx.z * 64 + lane;
-    const int p0 = (blockIdx.x * NW + wave) * P;
-    if (p0 >= PX) return;  // whole wave; no block-level barriers follow
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = uni(tid >> 6);
 
-    const AngleK k = make_angle(angles[a], scale, inv_scale);
-    const float cc = k.chi + k.clo;
-    const float ss = k.shi + k.slo;
-    const bool modeX = fabsf(cc) >= fabsf(ss);
+    const int nb = gridDim.x;
+    const int bid = blockIdx.x;
+    const int per = nb / kXcds;
+    const int logical = (bid < per * kXcds) ? (bid % kXcds) * per + bid / kXcds : bid;
+    const int ba = logical % nba;
+    const int bp = (logical / nba) % nbp;
+    const int bh = logical / (nba * nbp);
+
+    const int a = ba * NW + wave;
+    const int hb = bh * VOX;                     // first slice of the block
+    const int p0 = bp * P;
+    const int pl = p0 - 1;
+    const int ph = p0 + P;
     const float xc = 0.5f * (float)(SX - 1);
     const float yc = 0.5f * (float)(SY - 1);
     const float pc = 0.5f * (float)(PX - 1);
-    const int nrows = modeX ? SY : SX;
-    const int nalong = modeX ? SX : SY;
-    const float kap = modeX ? cc : ss;
-    const float oth = modeX ? ss : cc;
-    const float ca = modeX ? xc : yc;  // centre along the line
-    const float cr = modeX ? yc : xc;  // centre across lines
-    const int64_t st_along = modeX ? (int64_t)H : (int64_t)SX * H;
-    const int64_t st_row = modeX ? (int64_t)SX * H : (int64_t)H;
-    const int pl = p0 - 1;   // smallest centre pixel touching [p0, p0+P)
-    const int ph = p0 + P;   // largest
-    const float inv_kap = 1.0f / kap;
-    const int hc = min(h, H - 1);  // clamped slice for loads of idle lanes
+    const bool vec4 = (H & 3) == 0;
 
-    float acc[P], part[P];
+    if (tid < NW) {
+        const int at = ba * NW + tid;
+        const AngleK kk = make_angle(angles[min(at, A - 1)], scale, inv_scale);
+        s_k[tid] = kk;
+        const float cc = kk.chi + kk.clo, ss = kk.shi + kk.slo;
+        s_mode[tid] = (at < A) ? (fabsf(cc) >= fabsf(ss) ? 0 : 1) : -1;
+    }
+    __syncthreads();
+    AngleK k = s_k[wave];
+    k.chi = uni(k.chi); k.clo = uni(k.clo); k.shi = uni(k.shi); k.slo = uni(k.slo);
+    k.a = uni(k.a); k.b = uni(k.b); k.h = uni(k.h); k.cd = uni(k.cd);
+    const int mode = uni(s_mode[wave]);
+    // Lane w < NW describes angle w of the block (for the staging union).
+    const AngleK kl = s_k[lane < NW ? lane : 0];
+    const int model = (lane < NW) ? s_mode[lane] : -1;
+
+    float acc[P][HL], part[P][HL];
 #pragma unroll
-    for (int i = 0; i < P; ++i) acc[i] = part[i] = 0.0f;
-
-    for (int row = 0; row < nrows; ++row) {
-        // Along-line coordinate j with u(j) ~ ub + j*kap.
-        const float R = (float)row - cr;
-        const float ub = fmaf(-ca, kap, fmaf(R, oth, pc));
-        const float ja = ((float)pl - 1.5f - ub) * inv_kap;
-        const float jb = ((float)ph + 1.5f - ub) * inv_kap;
-        const int jlo = max(0, (int)floorf(fminf(ja, jb)) - 1);
-        const int jhi = min(nalong - 1, (int)ceilf(fmaxf(ja, jb)) + 1);
-        const int n = jhi - jlo + 1;
-        if (n <= 0) continue;  // uniform
-
-        // Lane m: m-th voxel in increasing-u order.
-        const int m = lane;
-        const int jm = kap >= 0.0f ? jlo + min(m, n - 1) : jhi - min(m, n - 1);
-        const float Xv = modeX ? (float)jm - xc : R;
-        const float Yv = modeX ? R : (float)jm - yc;
-        const Taps tp = voxel_taps(Xv, Yv, pc, k);
-        const int r = (m < n) ? tp.r : 0x3fffffff;
-
-        // Stage the segment (all 64 slices of each voxel) in this wave's LDS.
-        const float* xrow = x + (int64_t)row * st_row + hc;
-        for (int b = 0; b < n; b += 8) {
+    for (int i = 0; i < P; ++i)
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int jj = rdlane(jm, min(b + i, n - 1));
-                s_x[wave][b + i][lane] = xrow[(int64_t)jj * st_along];
+        for (int j = 0; j < HL; ++j) acc[i][j] = part[i][j] = 0.0f;
+
+    for (int pass = 0; pass < 2; ++pass) {
+        const bool any = __ballot(model == pass) != 0ull;
+        if (!any) continue;  // block-uniform: every wave sees the same modes
+        const bool mine = (mode == pass);
+        const bool px_ = (pass == 0);
+        const int nrows = px_ ? SY : SX;
+        const int nalong = px_ ? SX : SY;
+        const float ca = px_ ? xc : yc;
+        const float cr = px_ ? yc : xc;
+        const int64_t st_along = px_ ? (int64_t)H : (int64_t)SX * H;
+        const int64_t st_row = px_ ? (int64_t)SX * H : (int64_t)H;
+
+        // Segment [jlo, jhi] of a line for an angle (kap, oth): voxels whose
+        // centre pixel can lie in [pl, ph] (one pixel of margin each side).
+        auto segment = [&](int row, float kap, float oth, int& jlo, int& jhi) {
+            const float R = (float)row - cr;
+            const float ub = fmaf(-ca, kap, fmaf(R, oth, pc));
+            const float ikap = 1.0f / kap;
+            const float ja = ((float)pl - 1.5f - ub) * ikap;
+            const float jb = ((float)ph + 1.5f - ub) * ikap;
+            jlo = max(0, (int)floorf(fminf(ja, jb)) - 1);
+            jhi = min(nalong - 1, (int)ceilf(fmaxf(ja, jb)) + 1);
+        };
+        // Union of the block's segments on a line (identical in every wave).
+        auto line_union = [&](int row, int& ulo, int& uhi) {
+            int lo = 0x7fffffff, hi = -1;
+            if (model == pass) {
+                const float cck = kl.chi + kl.clo, ssk = kl.shi + kl.slo;
+                int l, h2;
+                segment(row, px_ ? cck : ssk, px_ ? ssk : cck, l, h2);
+                if (h2 >= l) { lo = l; hi = h2; }
             }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-        int mm = popc64(__ballot(r < pl));
-        const int mend = popc64(__ballot(r <= ph));
 #pragma unroll
-        for (int q = 0; q < P + 2; ++q) {
-#pragma unroll
-            for (int rep = 0; rep < 2; ++rep) {
-                if (mm < mend && rdlane(r, mm) == pl + q) {
-                    const float xv = s_x[wave][mm][lane];
-                    const float wm = rdlane(tp.wm, mm);
-                    const float w0 = rdlane(tp.w0, mm);
-                    const float wp = rdlane(tp.wp, mm);
-                    if (q - 2 >= 0 && q - 2 < P) part[q - 2] = fmaf(wm, xv, part[q - 2]);
-                    if (q - 1 >= 0 && q - 1 < P) part[q - 1] = fmaf(w0, xv, part[q - 1]);
-                    if (q < P) part[q] = fmaf(wp, xv, part[q]);
-                    ++mm;
+            for (int o = 1; o < 8; o <<= 1) {
+                lo = min(lo, __shfl_xor(lo, o));
+                hi = max(hi, __shfl_xor(hi, o));
+            }
+            ulo = uni(lo);
+            uhi = uni(hi);
+        };
+        // First piece at or after line `row`.
+        auto first_task = [&](int row) {
+            Task t{-1, 0, -1};
+            for (; row < nrows; ++row) {
+                int ulo, uhi;
+                line_union(row, ulo, uhi);
+                if (uhi >= ulo) {
+                    t.row = row;
+                    t.j0 = ulo;
+                    t.j1 = min(uhi, ulo + UMAX - 1);
+                    break;
                 }
             }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-        if ((row % kRowsPerPart) == kRowsPerPart - 1) {
+            return t;
+        };
+        auto next_task = [&](const Task& t) {
+            int ulo, uhi;
+            line_union(t.row, ulo, uhi);
+            if (t.j1 < uhi) return Task{t.row, t.j1 + 1, min(uhi, t.j1 + UMAX)};
+            return first_task(t.row + 1);
+        };
+        float4 reg[PER];
+        auto fetch = [&](const Task& t) {
+            const int nv4 = (t.j1 - t.j0 + 1) * (VOX / 4);
+            const float* base = x + (int64_t)t.row * st_row + (int64_t)t.j0 * st_along;
 #pragma unroll
-            for (int i = 0; i < P; ++i) {
-                acc[i] += part[i];
-                part[i] = 0.0f;
+            for (int i = 0; i < PER; ++i) {
+                const int e = tid + i * NT;
+                if (e < nv4) {
+                    const int v = e / (VOX / 4);
+                    const int hq = (e % (VOX / 4)) * 4;
+                    const float* src = base + (int64_t)v * st_along;
+                    const int hh = hb + hq;
+                    if (vec4 && hh + 3 < H) {
+                        reg[i] = *reinterpret_cast<const float4*>(src + hh);
+                    } else {
+                        reg[i] = make_float4(src[min(hh, H - 1)], src[min(hh + 1, H - 1)],
+                                             src[min(hh + 2, H - 1)], src[min(hh + 3, H - 1)]);
+                    }
+                }
             }
+        };
+        auto store = [&](int buf, const Task& t) {
+            const int nv4 = (t.j1 - t.j0 + 1) * (VOX / 4);
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                const int e = tid + i * NT;
+                if (e < nv4) s_x4[buf][e] = reg[i];
+            }
+        };
+
+        __syncthreads();  // previous pass's readers of s_x4 are done
+        Task cur = first_task(0);
+        if (cur.row >= 0) {
+            fetch(cur);
+            store(0, cur);
+        }
+        __syncthreads();
+        int buf = 0;
+        int taps_row = -1;
+        Taps tp{0, 0.f, 0.f, 0.f};
+        int r = 0, rprev_l = 0, jlo = 0, jhi = -1, mm0 = 0, mend = 0;
+        bool fwd = true;
+        int rows_done = 0;
+        while (cur.row >= 0) {
+            const Task nxt = next_task(cur);
+            if (nxt.row >= 0) fetch(nxt);
+            if (cur.row != taps_row) {
+                // New line: fold partial sums, then this wave's footprints.
+                for (; rows_done < cur.row; ++rows_done) {
+                    if ((rows_done % kRowsPerPart) == kRowsPerPart - 1) {
+#pragma unroll
+                        for (int i = 0; i < P; ++i)
+#pragma unroll
+                            for (int j = 0; j < HL; ++j) {
+                                acc[i][j] += part[i][j];
+                                part[i][j] = 0.0f;
+                            }
+                    }
+                }
+                taps_row = cur.row;
+                mend = mm0 = 0;
+                if (mine) {
+                    const float kap = px_ ? (k.chi + k.clo) : (k.shi + k.slo);
+                    const float oth = px_ ? (k.shi + k.slo) : (k.chi + k.clo);
+                    fwd = kap >= 0.0f;
+                    segment(cur.row, kap, oth, jlo, jhi);
+                    jlo = uni(jlo);
+                    jhi = uni(jhi);
+                    const int n = jhi - jlo + 1;
+                    if (n > 0) {
+                        const float R = (float)cur.row - cr;
+                        const int mcl = min(lane, n - 1);
+                        const int jm = fwd ? jlo + mcl : jhi - mcl;
+                        const int jp = fwd ? jm - 1 : jm + 1;
+                        tp = voxel_taps(px_ ? (float)jm - xc : R, px_ ? R : (float)jm - yc, pc, k);
+                        rprev_l =
+                            voxel_r(px_ ? (float)jp - xc : R, px_ ? R : (float)jp - yc, pc, k);
+
+                        const bool valid = lane < n;
+                        r = valid ? tp.r : 0x3fffffff;
+                        mm0 = uni((int)__popcll(__ballot(valid && r < pl)));
+                        mend = uni((int)__popcll(__ballot(valid && r <= ph)));
+                    }
+                }
+            }
+            if (mend > mm0) {
+                int mA, mB;
+                if (fwd) {
+                    mA = cur.j0 - jlo;
+                    mB = cur.j1 - jlo;
+                } else {
+                    mA = jhi - cur.j1;
+                    mB = jhi - cur.j0;
+                }
+                mA = uni(max(mA, mm0));
+                mB = uni(min(mB, mend - 1));
+                if (mA <= mB) {
+                    // Scatter this piece's footprints into the slot table:
+                    // slot (rank, q) holds the rank-th voxel whose centre pixel
+                    // is pl + q (at most two per pixel); empty slots keep zero
+                    // weights and a valid dummy offset, so the sweep below is
+                    // straight-line code.
+                    float4* tab = &s_t[wave][0][0];
+                    for (int i = lane; i < 2 * (P + 2); i += 64) tab[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    const int jb0 = fwd ? (jlo - cur.j0) : (jhi - cur.j0);
+                    const int dj = fwd ? 1 : -1;
+                    if (lane >= mA && lane <= mB) {
+                        const int q = r - pl;
+                        const int rank = (lane > mA && r == rprev_l) ? 1 : 0;
+                        const int off = (jb0 + dj * lane) * VOX;
+                        tab[rank * (P + 2) + q] =
+                            make_float4(tp.wm, tp.w0, tp.wp, __builtin_bit_cast(float, off));
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    const int ql = min(lane, P + 1);
+                    const float4 t0 = tab[ql];
+                    const float4 t1 = tab[(P + 2) + ql];
+                    const float* sx = reinterpret_cast<const float*>(s_x4[buf]) + lane * HL;
+                    auto ld = [&](float offbits) {
+                        const float* src = sx + __builtin_bit_cast(int, offbits);
+                        struct V { float a[HL]; } out;
+                        if constexpr (HL == 2) {
+                            const float2 t2 = *reinterpret_cast<const float2*>(src);
+                            out.a[0] = t2.x;
+                            out.a[1] = t2.y;
+                        } else {
+#pragma unroll
+                            for (int j = 0; j < HL; ++j) out.a[j] = src[j];
+                        }
+                        return out;
+                    };
+#pragma unroll
+                    for (int q = 0; q < P + 2; ++q) {
+#pragma unroll
+                        for (int rk = 0; rk < 2; ++rk) {
+                            const float4& t = rk == 0 ? t0 : t1;
+                            const auto xv = ld(rdlane(t.w, q));
+                            const float wm = rdlane(t.x, q);
+                            const float w0 = rdlane(t.y, q);
+                            const float wp = rdlane(t.z, q);
+#pragma unroll
+                            for (int j = 0; j < HL; ++j) {
+                                if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wm, xv.a[j], part[q - 2][j]);
+                                if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(w0, xv.a[j], part[q - 1][j]);
+                                if (q < P) part[q][j] = fmaf(wp, xv.a[j], part[q][j]);
+                            }
+                        }
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                }
+            }
+            if (nxt.row >= 0) store(buf ^ 1, nxt);
+            __syncthreads();
+            buf ^= 1;
+            cur = nxt;
         }
     }
 #pragma unroll
-    for (int i = 0; i < P; ++i) acc[i] += part[i];
-
-    if (h < H) {
+    for (int i = 0; i < P; ++i)
 #pragma unroll
-        for (int i = 0; i < P; ++i) {
-            const int p = p0 + i;
-            if (p < PX) {
-                float* py = y + ((int64_t)a * PX + p) * H + h;
-                *py = accumulate ? (*py + acc[i]) : acc[i];
+        for (int j = 0; j < HL; ++j) acc[i][j] += part[i][j];
+
+    if (a < A) {
+#pragma unroll
+        for (int j = 0; j < HL; ++j) {
+            const int hh = hb + lane * HL + j;
+            if (hh < H) {
+#pragma unroll
+                for (int i = 0; i < P; ++i) {
+                    const int p = p0 + i;
+                    if (p < PX) {
+                        float* py = y + ((int64_t)a * PX + p) * H + hh;
+                        *py = accumulate ? (*py + acc[i][j]) : acc[i][j];
+                    }
+                }
             }
         }
     }
@@ -180,13 +389,17 @@ __global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict_
 hipError_t launch_forward_z0(const float* x, float* y, const float* angles, const Geom& g,
                              int accumulate, hipStream_t s)
 {
-    constexpr int P = 32, NW = 4;
+    constexpr int P = 32, NW = 8, HL = 2, UMAX = 72;
     const double scale = ldexp(1.0, g.bits);
     const double inv_scale = ldexp(1.0, -g.bits);
-    dim3 grid((unsigned)((g.PX + P * NW - 1) / (P * NW)), (unsigned)g.A,
-              (unsigned)((g.H + 63) / 64));
-    hipLaunchKernelGGL((forward_z0_kernel<P, NW>), grid, dim3(64 * NW), 0, s, x, y, angles, g.A,
-                       (int)g.SY, (int)g.SX, (int)g.H, (int)g.PX, scale, inv_scale, accumulate);
+    const int nbp = (int)((g.PX + P - 1) / P);
+    const int nba = (g.A + NW - 1) / NW;
+    const int nbh = (int)((g.H + 64 * HL - 1) / (64 * HL));
+    const int64_t nb = (int64_t)nbp * nba * nbh;
+    if (nb > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((forward_z0_kernel<P, NW, HL, UMAX>), dim3((unsigned)nb), dim3(64 * NW), 0, s, x,
+                       y, angles, g.A, (int)g.SY, (int)g.SX, (int)g.H, (int)g.PX, scale, inv_scale,
+                       accumulate, nbp, nba);
     return hipGetLastError();
 }
 
