@@ -1,73 +1,90 @@
-// Voxel-driven backprojection x += AT y (reference: sp_backward_z0.cuh:8-100,
+// Voxel-driven backprojection x (+)= AT y (reference: sp_backward_z0.cuh:8-100,
 // sp_backward_z2.cuh:8-101), gather form, no atomics, deterministic.
 //
-// Work decomposition
-//   block  = TX x TY voxel tile x one chunk of 64 consecutive H slices;
-//   wave   = 64 voxels of the tile; lane = H slice (so every weight is
-//            wave-uniform and each sinogram read is one 256-byte row);
-//   angles = processed in chunks of AC.  Per chunk the sinogram window that
-//            the tile touches (WIN pixels x 64 slices per angle) is staged in
-//            LDS, so any number of angles works (the reference's 512-angle
-//            shared-memory table limit does not exist here).
-// Weights: lane v computes the footprint of voxel v for the current angle
-// (one evaluation per voxel-angle, shared by all 64 slices), and the 64
-// voxel iterations pick them up with v_readlane into SGPRs.
+// Work decomposition (z_order 0: H fastest)
+//   block  = TX x TY voxel tile x 64*HL consecutive slices; NW waves;
+//   wave   = VPW voxels of the tile; lane = HL consecutive slices, so every
+//            footprint weight is wave-uniform and each sinogram read is one
+//            contiguous 256*HL-byte row;
+//   angles = chunks of AC.  The sinogram window that the tile touches
+//            (WIN pixels x 64*HL slices per angle) is staged in LDS by
+//            global->LDS DMA, double-buffered: chunk i+1 streams in while
+//            chunk i is consumed, one barrier per chunk.  Any number of
+//            angles works (no 512-angle shared-memory table).
+// Footprints: lanes evaluate (voxel, angle) pairs in parallel -- VPW voxels x
+// 64/VPW angles per instruction -- and the voxel loop picks the weights up
+// with v_readlane into SGPRs.  Taps that fall off the detector get weight 0
+// (the reference reads 0 there through border addressing), so the DMA may
+// fetch a clamped, valid pixel row instead.
+// Summation: per chunk in registers, then into a second register set, so the
+// FP32 sum over angles is two-level.
 #include "vp_device.hpp"
 #include "vp_launch.hpp"
 
 namespace vp {
 namespace {
 
+constexpr int kXcds = 8;  // MI355X dispatches workgroup i to XCD i % 8
+
 __device__ __forceinline__ float rdlane(float v, int l)
 {
     return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
 }
 __device__ __forceinline__ int rdlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-constexpr int kRow = 65;  // LDS row: 64 slices + 1 pad word (conflict-free transposed stores)
-
-template <int Z, int TX, int TY, int AC, int WIN>
-__global__ __launch_bounds__(TX * TY) void backproject_kernel(
+template <int TX, int TY, int NW, int AC, int WIN, int HL>
+__global__ __launch_bounds__(64 * NW) void backproject_z0_kernel(
     const float* __restrict__ y, float* __restrict__ x, const float* __restrict__ angles,
-    int A, int SY, int SX, int H, int PX, double scale, double inv_scale)
+    int A, int SY, int SX, int H, int PX, double scale, double inv_scale, int accumulate,
+    int nbx, int nby)
 {
-    constexpr int NT = TX * TY;
-    constexpr int NW = NT / 64;
-    static_assert(NT % 64 == 0, "tile must hold whole waves");
-    constexpr int STAGE = AC * WIN * kRow;
-    constexpr int OUTT = 64 * kRow;
-    constexpr int LDSN = STAGE > OUTT ? STAGE : OUTT;
-    __shared__ float s_y[LDSN];
-    __shared__ AngleK s_k[AC];
-    __shared__ int s_rlo[AC];
+    constexpr int VPW = TX * TY / NW;        // voxels per wave
+    constexpr int APL = 64 / VPW;            // angles evaluated per lane sweep
+    constexpr int VOX = 64 * HL;             // floats per staged pixel row
+    constexpr int STAGE = AC * WIN * VOX;    // floats per buffer
+    static_assert(VPW * APL == 64 && AC % APL == 0, "lane layout");
+    static_assert(1024 % (VOX * 4) == 0 && (AC * WIN) % (1024 / (VOX * 4)) == 0,
+                  "one 16-byte-per-lane DMA instruction must cover whole pixel rows");
+    __shared__ float4 s_y4[2][STAGE / 4];
+    __shared__ AngleK s_k[3][AC];
+    __shared__ int s_rlo[3][AC];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
-    const int tx0 = blockIdx.x * TX;
-    const int ty0 = blockIdx.y * TY;
-    const int h0 = blockIdx.z * 64;
+    const int wave = uni(tid >> 6);
+
+    // XCD-aware order: XCD k runs a contiguous range of logical tiles, so
+    // neighbouring tiles (which share sinogram rows) meet in the same L2.
+    const int nb = gridDim.x;
+    const int bid = blockIdx.x;
+    const int per = nb / kXcds;
+    const int logical = (bid < per * kXcds) ? (bid % kXcds) * per + bid / kXcds : bid;
+    const int bx = logical % nbx;
+    const int by = (logical / nbx) % nby;
+    const int bhc = logical / (nbx * nby);
+    const int tx0 = bx * TX, ty0 = by * TY;
+    const int hb = bhc * VOX;
     const float xc = 0.5f * (float)(SX - 1);
     const float yc = 0.5f * (float)(SY - 1);
     const float pc = 0.5f * (float)(PX - 1);
+    const bool vec4 = (H & 3) == 0;
 
-    // Voxel whose footprint this lane evaluates (wave-local index = lane).
-    const int vi = wave * 64 + lane;
-    const int vx = tx0 + vi % TX;
-    const int vy = ty0 + vi / TX;
-    const float X = (float)vx - xc;
-    const float Y = (float)vy - yc;
+    // Footprint lane layout: voxel (lane % VPW) of this wave, angle lane / VPW.
+    const int lv = lane % VPW;
+    const int la = lane / VPW;
+    const int vi = wave * VPW + lv;
+    const float X = (float)(tx0 + vi % TX) - xc;
+    const float Y = (float)(ty0 + vi / TX) - yc;
 
-    float acc[64];
-#pragma unroll
-    for (int v = 0; v < 64; ++v) acc[v] = 0.0f;
+    const int nchunks = (A + AC - 1) / AC;
 
-    for (int c0 = 0; c0 < A; c0 += AC) {
-        __syncthreads();
+    // Per-chunk angle constants and window base, written by threads < AC.
+    auto tables = [&](int chunk, int slot) {
         if (tid < AC) {
-            const int a = c0 + tid;
-            const AngleK k = make_angle(a < A ? angles[a] : 0.0f, scale, inv_scale);
-            s_k[tid] = k;
+            const int a = chunk * AC + tid;
+            const AngleK k = make_angle(angles[min(a, A - 1)], scale, inv_scale);
+            s_k[slot][tid] = k;
             const float xa = (float)tx0 - xc;
             const float xb = (float)(min(tx0 + TX, SX) - 1) - xc;
             const float ya = (float)ty0 - yc;
@@ -76,120 +93,161 @@ __global__ __launch_bounds__(TX * TY) void backproject_kernel(
             r = min(r, voxel_r(xb, ya, pc, k));
             r = min(r, voxel_r(xa, yb, pc, k));
             r = min(r, voxel_r(xb, yb, pc, k));
-            s_rlo[tid] = r - 2;  // one spare pixel below r-1 for rounding
+            s_rlo[slot][tid] = r - 2;  // one spare pixel below r-1 for rounding
         }
-        __syncthreads();
-        for (int e = tid; e < AC * WIN * 64; e += NT) {
-            int a, k, hl;
-            if (Z == 0) {
-                hl = e & 63;
-                k = (e >> 6) % WIN;
-                a = e / (64 * WIN);
-            } else {
-                k = e % WIN;
-                hl = (e / WIN) & 63;
-                a = e / (64 * WIN);
+    };
+    // DMA of a chunk's windows: pixel rows [rlo, rlo+WIN) of each angle,
+    // clamped to the detector (out-of-detector taps carry zero weight).
+    auto fetch = [&](int chunk, int slot, int b) {
+        float* dst0 = reinterpret_cast<float*>(s_y4[b]);
+        if (vec4) {
+            constexpr int RPI = 1024 / (VOX * 4);  // pixel rows per DMA instruction
+            constexpr int LPR = 64 / RPI;                                  // lanes per row
+            const int sub = lane / LPR;
+            const int hq = (lane % LPR) * 4;
+            const int hoff = min(hb + hq, H - 4);
+            for (int ins = wave; ins < AC * WIN / RPI; ins += NW) {
+                const int row = ins * RPI + sub;  // = a * WIN + k
+                const int a = row / WIN;
+                const int kk = row % WIN;
+                const int ag = min(chunk * AC + a, A - 1);
+                const int p = min(max(s_rlo[slot][a] + kk, 0), PX - 1);
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)(y + ((int64_t)ag * PX + p) * H + hoff),
+                    (__attribute__((address_space(3))) void*)(dst0 + ins * RPI * VOX), 16, 0, 0);
             }
-            const int ag = c0 + a;
-            const int p = s_rlo[a] + k;
-            const int h = h0 + hl;
-            float v = 0.0f;
-            if (ag < A && p >= 0 && p < PX && h < H) {
-                v = (Z == 0) ? y[((int64_t)ag * PX + p) * H + h]
-                             : y[((int64_t)h * A + ag) * PX + p];
+        } else {
+            for (int ins = wave; ins < AC * WIN * HL; ins += NW) {
+                const int row = ins / HL;
+                const int c = ins % HL;
+                const int a = row / WIN;
+                const int kk = row % WIN;
+                const int ag = min(chunk * AC + a, A - 1);
+                const int p = min(max(s_rlo[slot][a] + kk, 0), PX - 1);
+                const int hh = min(hb + 64 * c + lane, H - 1);
+                __builtin_amdgcn_global_load_lds(
+                    (const void*)(y + ((int64_t)ag * PX + p) * H + hh),
+                    (__attribute__((address_space(3))) void*)(dst0 + row * VOX + 64 * c), 4, 0, 0);
             }
-            s_y[(a * WIN + k) * kRow + hl] = v;
         }
-        __syncthreads();
+    };
 
-        float part[64];
+    float acc[VPW][HL];
 #pragma unroll
-        for (int v = 0; v < 64; ++v) part[v] = 0.0f;
-        const int na = min(AC, A - c0);
-        for (int a = 0; a < na; ++a) {
-            const AngleK k = s_k[a];
+    for (int v = 0; v < VPW; ++v)
+#pragma unroll
+        for (int j = 0; j < HL; ++j) acc[v][j] = 0.0f;
+
+    tables(0, 0);
+    if (nchunks > 1) tables(1, 1);
+    __syncthreads();
+    fetch(0, 0, 0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+
+    for (int c = 0; c < nchunks; ++c) {
+        const int b = c & 1;
+        const int slot = c % 3;
+        if (c + 1 < nchunks) fetch(c + 1, (c + 1) % 3, b ^ 1);
+        if (c + 2 < nchunks) tables(c + 2, (c + 2) % 3);
+
+        float part[VPW][HL];
+#pragma unroll
+        for (int v = 0; v < VPW; ++v)
+#pragma unroll
+            for (int j = 0; j < HL; ++j) part[v][j] = 0.0f;
+        const float* sy = reinterpret_cast<const float*>(s_y4[b]) + lane * HL;
+
+#pragma unroll
+        for (int ag = 0; ag < AC; ag += APL) {
+            // Footprint of (voxel lv, angle ag + la) in this lane.
+            const int al = ag + la;
+            const AngleK k = s_k[slot][al];
             const Taps t = voxel_taps(X, Y, pc, k);
-            int off = t.r - 1 - s_rlo[a];
+            const bool aval = c * AC + al < A;
+            const int p0 = t.r - 1;
+            const float wm = (aval && p0 >= 0 && p0 < PX) ? t.wm : 0.0f;
+            const float w0 = (aval && p0 + 1 >= 0 && p0 + 1 < PX) ? t.w0 : 0.0f;
+            const float wp = (aval && p0 + 2 >= 0 && p0 + 2 < PX) ? t.wp : 0.0f;
+            int off = p0 - s_rlo[slot][al];
             off = min(max(off, 0), WIN - 3);
-            const float* base = s_y + a * WIN * kRow + lane;
+            const int foff = (al * WIN + off) * VOX;  // LDS float offset of pixel r-1
 #pragma unroll
-            for (int v = 0; v < 64; ++v) {
-                const int o = rdlane(off, v);
-                const float wm = rdlane(t.wm, v);
-                const float w0 = rdlane(t.w0, v);
-                const float wp = rdlane(t.wp, v);
-                const float* pr = base + o * kRow;
-                float s = part[v];
-                s = fmaf(wm, pr[0], s);
-                s = fmaf(w0, pr[kRow], s);
-                s = fmaf(wp, pr[2 * kRow], s);
-                part[v] = s;
-            }
-        }
+            for (int s2 = 0; s2 < APL; ++s2) {
 #pragma unroll
-        for (int v = 0; v < 64; ++v) acc[v] += part[v];
-    }
-
-    const int h = h0 + lane;
-    if (Z == 0) {
-        if (h < H) {
+                for (int v = 0; v < VPW; ++v) {
+                    const int L = s2 * VPW + v;
+                    const float* pr = sy + rdlane(foff, L);
+                    const float a0 = rdlane(wm, L);
+                    const float a1 = rdlane(w0, L);
+                    const float a2 = rdlane(wp, L);
+                    if constexpr (HL == 2) {
+                        const float2 y0 = *reinterpret_cast<const float2*>(pr);
+                        const float2 y1 = *reinterpret_cast<const float2*>(pr + VOX);
+                        const float2 y2 = *reinterpret_cast<const float2*>(pr + 2 * VOX);
+                        part[v][0] = fmaf(a0, y0.x, part[v][0]);
+                        part[v][1] = fmaf(a0, y0.y, part[v][1]);
+                        part[v][0] = fmaf(a1, y1.x, part[v][0]);
+                        part[v][1] = fmaf(a1, y1.y, part[v][1]);
+                        part[v][0] = fmaf(a2, y2.x, part[v][0]);
+                        part[v][1] = fmaf(a2, y2.y, part[v][1]);
+                    } else {
 #pragma unroll
-            for (int v = 0; v < 64; ++v) {
-                const int wv = wave * 64 + v;
-                const int ix = tx0 + wv % TX;
-                const int iy = ty0 + wv / TX;
-                if (ix < SX && iy < SY) {
-                    float* px = x + ((int64_t)iy * SX + ix) * H + h;
-                    *px += acc[v];
+                        for (int j = 0; j < HL; ++j) {
+                            float s = part[v][j];
+                            s = fmaf(a0, pr[j], s);
+                            s = fmaf(a1, pr[VOX + j], s);
+                            s = fmaf(a2, pr[2 * VOX + j], s);
+                            part[v][j] = s;
+                        }
+                    }
                 }
             }
         }
-    } else {
-        // Transpose through LDS so the stores run along ix.
-        for (int w = 0; w < NW; ++w) {
-            __syncthreads();
-            if (wave == w) {
 #pragma unroll
-                for (int v = 0; v < 64; ++v) s_y[v * kRow + lane] = acc[v];
-            }
-            __syncthreads();
-            for (int e = tid; e < 64 * 64; e += NT) {
-                const int v = e & 63;
-                const int hl = e >> 6;
-                const int wv = w * 64 + v;
-                const int ix = tx0 + wv % TX;
-                const int iy = ty0 + wv / TX;
-                const int hh = h0 + hl;
-                if (ix < SX && iy < SY && hh < H) {
-                    float* px = x + ((int64_t)hh * SY + iy) * SX + ix;
-                    *px += s_y[v * kRow + hl];
+        for (int v = 0; v < VPW; ++v)
+#pragma unroll
+            for (int j = 0; j < HL; ++j) acc[v][j] += part[v][j];
+
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA into the other buffer landed
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int v = 0; v < VPW; ++v) {
+        const int wv = wave * VPW + v;
+        const int ix = tx0 + wv % TX;
+        const int iy = ty0 + wv / TX;
+        if (ix < SX && iy < SY) {
+#pragma unroll
+            for (int j = 0; j < HL; ++j) {
+                const int hh = hb + lane * HL + j;
+                if (hh < H) {
+                    float* px = x + ((int64_t)iy * SX + ix) * H + hh;
+                    *px = accumulate ? (*px + acc[v][j]) : acc[v][j];
                 }
             }
         }
     }
-}
-
-template <int Z>
-hipError_t launch_backward_t(const float* y, float* x, const float* angles, const Geom& g,
-                             hipStream_t s)
-{
-    constexpr int TX = 16, TY = 16, AC = 4, WIN = 28;
-    const double scale = ldexp(1.0, g.bits);
-    const double inv_scale = ldexp(1.0, -g.bits);
-    dim3 grid((unsigned)((g.SX + TX - 1) / TX), (unsigned)((g.SY + TY - 1) / TY),
-              (unsigned)((g.H + 63) / 64));
-    hipLaunchKernelGGL((backproject_kernel<Z, TX, TY, AC, WIN>), grid, dim3(TX * TY), 0, s, y, x,
-                       angles, g.A, (int)g.SY, (int)g.SX, (int)g.H, (int)g.PX, scale, inv_scale);
-    return hipGetLastError();
 }
 
 }  // namespace
 
-hipError_t launch_backward(int z_order, const float* y, float* x, const float* angles,
-                           const Geom& g, hipStream_t s)
+hipError_t launch_backward_z0(const float* y, float* x, const float* angles, const Geom& g,
+                              int accumulate, hipStream_t s)
 {
-    return z_order == 0 ? launch_backward_t<0>(y, x, angles, g, s)
-                        : launch_backward_t<2>(y, x, angles, g, s);
+    constexpr int TX = 16, TY = 16, NW = 8, AC = 4, WIN = 28, HL = 2;
+    const double scale = ldexp(1.0, g.bits);
+    const double inv_scale = ldexp(1.0, -g.bits);
+    const int nbx = (int)((g.SX + TX - 1) / TX);
+    const int nby = (int)((g.SY + TY - 1) / TY);
+    const int nbh = (int)((g.H + 64 * HL - 1) / (64 * HL));
+    const int64_t nb = (int64_t)nbx * nby * nbh;
+    if (nb > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((backproject_z0_kernel<TX, TY, NW, AC, WIN, HL>), dim3((unsigned)nb),
+                       dim3(64 * NW), 0, s, y, x, angles, g.A, (int)g.SY, (int)g.SX, (int)g.H,
+                       (int)g.PX, scale, inv_scale, accumulate, nbx, nby);
+    return hipGetLastError();
 }
 
 }  // namespace vp

@@ -99,8 +99,28 @@ int vp_backward(int z_order, const float* y, float* x, const float* angles, int 
     bool empty = false;
     int rc = validate(z_order, y, x, angles, n_angles, SY, SX, H, PX, &g, &empty);
     if (rc != VP_OK || empty) return rc;
-    hipError_t e = vp::launch_backward(z_order, y, x, angles, g, (hipStream_t)stream);
-    if (e != hipSuccess) return hip_fail(e, "backward launch");
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e;
+    if (z_order == 0) {
+        e = vp::launch_backward_z0(y, x, angles, g, 1, s);
+        if (e != hipSuccess) return hip_fail(e, "backward z_order 0 launch");
+        return VP_OK;
+    }
+    // z_order 2: transpose the sinogram to slice-fastest order, backproject,
+    // and add the transposed volume back.
+    const int64_t nx = SY * SX * H;
+    const int64_t ny = (int64_t)n_angles * PX * H;
+    float* ws = nullptr;
+    e = hipMallocAsync((void**)&ws, (size_t)(nx + ny) * sizeof(float), s);
+    if (e != hipSuccess) return hip_fail(e, "backward z_order 2 workspace");
+    float* x0 = ws;
+    float* y0 = ws + nx;
+    e = vp::launch_transpose(y, y0, H, (int64_t)n_angles * PX, 0, s);
+    if (e == hipSuccess) e = vp::launch_backward_z0(y0, x0, angles, g, 0, s);
+    if (e == hipSuccess) e = vp::launch_transpose(x0, x, SY * SX, H, 1, s);
+    hipError_t ef = hipFreeAsync(ws, s);
+    if (e != hipSuccess) return hip_fail(e, "backward z_order 2 launch");
+    if (ef != hipSuccess) return hip_fail(ef, "backward z_order 2 workspace release");
     return VP_OK;
 }
 

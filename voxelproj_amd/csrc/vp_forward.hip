@@ -27,7 +27,7 @@ __device__ __forceinline__ float rdlane(float v, int l)
 }
 
 constexpr int kSeg = 64;       // max voxels of one wave's line segment (one per lane)
-constexpr int kRowsPerPart = 16;
+constexpr int kRowsPerFold = 128;
 constexpr int kXcds = 8;       // MI355X dispatches workgroup i to XCD i % 8
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -57,10 +57,8 @@ __global__ __launch_bounds__(64 * NW) void forward_z0_kernel(
     int nbp, int nba)
 {
     static_assert((P + 5) * 1.4143 + 4 <= kSeg, "segment may exceed one wave");
-    constexpr int NT = 64 * NW;
     constexpr int VOX = 64 * HL;                 // floats per staged voxel
     constexpr int STAGE = UMAX * VOX;            // floats per buffer
-    constexpr int PER = (STAGE / 4 + NT - 1) / NT;  // float4 per thread per piece
     __shared__ float4 s_x4[2][STAGE / 4];
     __shared__ AngleK s_k[NW];
     __shared__ int s_mode[NW];
@@ -105,11 +103,38 @@ __global__ __launch_bounds__(64 * NW) void forward_z0_kernel(
     const AngleK kl = s_k[lane < NW ? lane : 0];
     const int model = (lane < NW) ? s_mode[lane] : -1;
 
-    float acc[P][HL], part[P][HL];
+    // Register partial sums over at most kRowsPerFold lines; the owning wave
+    // folds them into y (it is the only writer of its outputs), which keeps the
+    // FP32 summation two-level without a second register array.
+    float part[P][HL];
 #pragma unroll
     for (int i = 0; i < P; ++i)
 #pragma unroll
-        for (int j = 0; j < HL; ++j) acc[i][j] = part[i][j] = 0.0f;
+        for (int j = 0; j < HL; ++j) part[i][j] = 0.0f;
+    bool folded = false;
+    auto fold = [&]() {
+        if (a < A) {
+#pragma unroll
+            for (int j = 0; j < HL; ++j) {
+                const int hh = hb + lane * HL + j;
+                if (hh < H) {
+#pragma unroll
+                    for (int i = 0; i < P; ++i) {
+                        const int p = p0 + i;
+                        if (p < PX) {
+                            float* py = y + ((int64_t)a * PX + p) * H + hh;
+                            *py = (accumulate || folded) ? (*py + part[i][j]) : part[i][j];
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < P; ++i)
+#pragma unroll
+            for (int j = 0; j < HL; ++j) part[i][j] = 0.0f;
+        folded = true;
+    };
 
     for (int pass = 0; pass < 2; ++pass) {
         const bool any = __ballot(model == pass) != 0ull;
@@ -172,42 +197,41 @@ __global__ __launch_bounds__(64 * NW) void forward_z0_kernel(
             if (t.j1 < uhi) return Task{t.row, t.j1 + 1, min(uhi, t.j1 + UMAX)};
             return first_task(t.row + 1);
         };
-        float4 reg[PER];
-        auto fetch = [&](const Task& t) {
-            const int nv4 = (t.j1 - t.j0 + 1) * (VOX / 4);
+        // Stage a piece straight into LDS with global->LDS DMA: one wave
+        // instruction moves one voxel's 64*HL slices (16 B per lane when H is a
+        // multiple of 4, else HL dword instructions).  Idle lanes past H load a
+        // clamped, valid address; their results are never stored.
+        auto fetch = [&](const Task& t, int b) {
+            const int nv = t.j1 - t.j0 + 1;
             const float* base = x + (int64_t)t.row * st_row + (int64_t)t.j0 * st_along;
+            if (vec4) {
+                const int hoff = min(hb + lane * 4, H - 4);
+                for (int v = wave; v < nv; v += NW) {
+                    __builtin_amdgcn_global_load_lds(
+                        (const void*)(base + (int64_t)v * st_along + hoff),
+                        (__attribute__((address_space(3))) void*)&s_x4[b][v * (VOX / 4)], 16, 0, 0);
+                }
+            } else {
+                for (int v = wave; v < nv; v += NW) {
 #pragma unroll
-            for (int i = 0; i < PER; ++i) {
-                const int e = tid + i * NT;
-                if (e < nv4) {
-                    const int v = e / (VOX / 4);
-                    const int hq = (e % (VOX / 4)) * 4;
-                    const float* src = base + (int64_t)v * st_along;
-                    const int hh = hb + hq;
-                    if (vec4 && hh + 3 < H) {
-                        reg[i] = *reinterpret_cast<const float4*>(src + hh);
-                    } else {
-                        reg[i] = make_float4(src[min(hh, H - 1)], src[min(hh + 1, H - 1)],
-                                             src[min(hh + 2, H - 1)], src[min(hh + 3, H - 1)]);
+                    for (int c = 0; c < HL; ++c) {
+                        // dword instruction c fills floats [64c, 64c+64) of the
+                        // voxel's row, i.e. slices hb + 64c + lane.
+                        const int e = 64 * c + lane;           // float index in the row
+                        const int hh = min(hb + e, H - 1);
+                        __builtin_amdgcn_global_load_lds(
+                            (const void*)(base + (int64_t)v * st_along + hh),
+                            (__attribute__((address_space(3))) void*)(reinterpret_cast<float*>(&s_x4[b][v * (VOX / 4)]) + 64 * c),
+                            4, 0, 0);
                     }
                 }
-            }
-        };
-        auto store = [&](int buf, const Task& t) {
-            const int nv4 = (t.j1 - t.j0 + 1) * (VOX / 4);
-#pragma unroll
-            for (int i = 0; i < PER; ++i) {
-                const int e = tid + i * NT;
-                if (e < nv4) s_x4[buf][e] = reg[i];
             }
         };
 
         __syncthreads();  // previous pass's readers of s_x4 are done
         Task cur = first_task(0);
-        if (cur.row >= 0) {
-            fetch(cur);
-            store(0, cur);
-        }
+        if (cur.row >= 0) fetch(cur, 0);
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         int buf = 0;
         int taps_row = -1;
@@ -217,20 +241,12 @@ __global__ __launch_bounds__(64 * NW) void forward_z0_kernel(
         int rows_done = 0;
         while (cur.row >= 0) {
             const Task nxt = next_task(cur);
-            if (nxt.row >= 0) fetch(nxt);
+            if (nxt.row >= 0) fetch(nxt, buf ^ 1);
             if (cur.row != taps_row) {
-                // New line: fold partial sums, then this wave's footprints.
-                for (; rows_done < cur.row; ++rows_done) {
-                    if ((rows_done % kRowsPerPart) == kRowsPerPart - 1) {
-#pragma unroll
-                        for (int i = 0; i < P; ++i)
-#pragma unroll
-                            for (int j = 0; j < HL; ++j) {
-                                acc[i][j] += part[i][j];
-                                part[i][j] = 0.0f;
-                            }
-                    }
-                }
+                // New line: fold partial sums every kRowsPerFold lines, then
+                // this wave's footprints.
+                if (cur.row / kRowsPerFold != rows_done / kRowsPerFold) fold();
+                rows_done = cur.row;
                 taps_row = cur.row;
                 mend = mm0 = 0;
                 if (mine) {
@@ -294,11 +310,20 @@ __global__ __launch_bounds__(64 * NW) void forward_z0_kernel(
                     const int ql = min(lane, P + 1);
                     const float4 t0 = tab[ql];
                     const float4 t1 = tab[(P + 2) + ql];
+                    // bit q: pixel pl+q holds a second voxel (about 1 pixel in 4)
+                    const uint64_t has2 = __ballot(lane < P + 2 && (t1.x != 0.f || t1.y != 0.f || t1.z != 0.f));
+                    const int dvox = dj * VOX;
                     const float* sx = reinterpret_cast<const float*>(s_x4[buf]) + lane * HL;
-                    auto ld = [&](float offbits) {
-                        const float* src = sx + __builtin_bit_cast(int, offbits);
+                    auto ld = [&](int off) {
+                        const float* src = sx + off;
                         struct V { float a[HL]; } out;
-                        if constexpr (HL == 2) {
+                        if constexpr (HL == 4) {
+                            const float4 t4 = *reinterpret_cast<const float4*>(src);
+                            out.a[0] = t4.x;
+                            out.a[1] = t4.y;
+                            out.a[2] = t4.z;
+                            out.a[3] = t4.w;
+                        } else if constexpr (HL == 2) {
                             const float2 t2 = *reinterpret_cast<const float2*>(src);
                             out.a[0] = t2.x;
                             out.a[1] = t2.y;
@@ -310,13 +335,24 @@ __global__ __launch_bounds__(64 * NW) void forward_z0_kernel(
                     };
 #pragma unroll
                     for (int q = 0; q < P + 2; ++q) {
+                        const int off0 = __builtin_bit_cast(int, rdlane(t0.w, q));
+                        {
+                            const auto xv = ld(off0);
+                            const float wm = rdlane(t0.x, q);
+                            const float w0 = rdlane(t0.y, q);
+                            const float wp = rdlane(t0.z, q);
 #pragma unroll
-                        for (int rk = 0; rk < 2; ++rk) {
-                            const float4& t = rk == 0 ? t0 : t1;
-                            const auto xv = ld(rdlane(t.w, q));
-                            const float wm = rdlane(t.x, q);
-                            const float w0 = rdlane(t.y, q);
-                            const float wp = rdlane(t.z, q);
+                            for (int j = 0; j < HL; ++j) {
+                                if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wm, xv.a[j], part[q - 2][j]);
+                                if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(w0, xv.a[j], part[q - 1][j]);
+                                if (q < P) part[q][j] = fmaf(wp, xv.a[j], part[q][j]);
+                            }
+                        }
+                        if ((has2 >> q) & 1ull) {
+                            const auto xv = ld(off0 + dvox);
+                            const float wm = rdlane(t1.x, q);
+                            const float w0 = rdlane(t1.y, q);
+                            const float wp = rdlane(t1.z, q);
 #pragma unroll
                             for (int j = 0; j < HL; ++j) {
                                 if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wm, xv.a[j], part[q - 2][j]);
@@ -330,33 +366,14 @@ __global__ __launch_bounds__(64 * NW) void forward_z0_kernel(
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 }
             }
-            if (nxt.row >= 0) store(buf ^ 1, nxt);
+            __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA into buf^1 has landed
             __syncthreads();
             buf ^= 1;
             cur = nxt;
         }
+        fold();
     }
-#pragma unroll
-    for (int i = 0; i < P; ++i)
-#pragma unroll
-        for (int j = 0; j < HL; ++j) acc[i][j] += part[i][j];
-
-    if (a < A) {
-#pragma unroll
-        for (int j = 0; j < HL; ++j) {
-            const int hh = hb + lane * HL + j;
-            if (hh < H) {
-#pragma unroll
-                for (int i = 0; i < P; ++i) {
-                    const int p = p0 + i;
-                    if (p < PX) {
-                        float* py = y + ((int64_t)a * PX + p) * H + hh;
-                        *py = accumulate ? (*py + acc[i][j]) : acc[i][j];
-                    }
-                }
-            }
-        }
-    }
+    fold();
 }
 
 constexpr int kTile = 64;
@@ -389,7 +406,7 @@ __global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict_
 hipError_t launch_forward_z0(const float* x, float* y, const float* angles, const Geom& g,
                              int accumulate, hipStream_t s)
 {
-    constexpr int P = 32, NW = 8, HL = 2, UMAX = 72;
+    constexpr int P = 32, NW = 8, HL = 4, UMAX = 72;
     const double scale = ldexp(1.0, g.bits);
     const double inv_scale = ldexp(1.0, -g.bits);
     const int nbp = (int)((g.PX + P - 1) / P);

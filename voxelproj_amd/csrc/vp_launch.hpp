@@ -12,9 +12,9 @@ struct Geom {
     int bits;           // fractional bits of the cos/sin hi split
 };
 
-// x += AT y, z_order 0 or 2.
-hipError_t launch_backward(int z_order, const float* y, float* x, const float* angles,
-                           const Geom& g, hipStream_t s);
+// x (+)= AT y in z_order 0 (accumulate != 0 adds, else overwrites).
+hipError_t launch_backward_z0(const float* y, float* x, const float* angles, const Geom& g,
+                              int accumulate, hipStream_t s);
 
 // y (+)= A x in z_order 0 (accumulate != 0 adds, else overwrites).
 hipError_t launch_forward_z0(const float* x, float* y, const float* angles, const Geom& g,
