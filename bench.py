@@ -63,8 +63,10 @@ def time_op(fn, args, n, stream):
     return start.elapsed_time(end) / n
 
 
-def cpu_baseline(N, A, H_sample):
-    """Reference algorithm (FP32 transcription, OpenMP) on H_sample slices."""
+def cpu_baseline(N, A, H_sample, budget_s=10.0):
+    """Reference algorithm (FP32 transcription, OpenMP) on H_sample slices of
+    the workload, repeated until about budget_s seconds of CPU time are spent;
+    the rate is taken from the best repetition."""
     import oracle as vpo
 
     g = torch.Generator().manual_seed(5)
@@ -72,20 +74,28 @@ def cpu_baseline(N, A, H_sample):
     x = torch.randn((N, N, H_sample), generator=g).numpy()
     y = torch.randn((A, N, H_sample), generator=g).numpy()
     vpo.ref_forward_f32(x[:, :, :1], ang, N, 0)  # warm up threads / pages
-    t0 = time.perf_counter()
-    vpo.ref_forward_f32(x, ang, N, 0)
-    t1 = time.perf_counter()
-    vpo.ref_backward_f32(y, ang, N, N, 0)
-    t2 = time.perf_counter()
+    best = None
+    spent = 0.0
+    reps = 0
+    while spent < budget_s or reps < 1:
+        t0 = time.perf_counter()
+        vpo.ref_forward_f32(x, ang, N, 0)
+        t1 = time.perf_counter()
+        vpo.ref_backward_f32(y, ang, N, N, 0)
+        t2 = time.perf_counter()
+        spent += t2 - t0
+        reps += 1
+        if best is None or t2 - t0 < best[0]:
+            best = (t2 - t0, t1 - t0, t2 - t1)
     work = 2.0 * N * N * H_sample * A
     return {
-        "value": round(work / (t2 - t0) / 1e9, 3),
+        "value": round(work / best[0] / 1e9, 3),
         "unit": "Gvoxel*angle/s",
         "cores": vpo.max_threads(),
         "kind": "port",
-        "sample": (f"A then AT, FP32 transcription of the reference kernels, "
-                   f"{N}x{N}x{H_sample} slices x {A} angles (z_order 0), "
-                   f"A {t1 - t0:.2f} s, AT {t2 - t1:.2f} s"),
+        "sample": (f"A then AT, FP32 transcription of the reference kernels (oracle/), "
+                   f"{N}x{N}x{H_sample} slices x {A} angles (z_order 0), {reps} repetitions "
+                   f"in {spent:.1f} s; best A {best[1]:.2f} s, AT {best[2]:.2f} s"),
     }
 
 
@@ -97,7 +107,7 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--angles", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-slices", type=int, default=128)
+    ap.add_argument("--cpu-slices", type=int, default=256)
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3 counters: one 512 MiB copy (reference "
                          "traffic), then one A and one AT launch; prints no JSON")

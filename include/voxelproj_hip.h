@@ -54,6 +54,17 @@ VP_API int vp_forward(int z_order, const float *x, float *y, const float *angles
 VP_API int vp_backward(int z_order, const float *y, float *x, const float *angles, int n_angles,
                 int64_t SY, int64_t SX, int64_t H, int64_t PX, void *stream);
 
+/*
+ * Double-precision variants of vp_forward / vp_backward (same layouts and
+ * accumulate semantics, float64 data, float32 angles).  No counterpart in
+ * the reference, which is float32-only (voxelproj/csrc/projectors.cu:37);
+ * they serve FP64 gradient checks and solvers.
+ */
+VP_API int vp_forward_f64(int z_order, const double *x, double *y, const float *angles, int n_angles,
+                          int64_t SY, int64_t SX, int64_t H, int64_t PX, void *stream);
+VP_API int vp_backward_f64(int z_order, const double *y, double *x, const float *angles, int n_angles,
+                           int64_t SY, int64_t SX, int64_t H, int64_t PX, void *stream);
+
 /* Description of the last failure on the calling thread ("" if none). */
 VP_API const char *vp_last_error(void);
 

@@ -12,7 +12,8 @@ import torch  # noqa: F401  -- loads torch's HIP runtime first, so the library b
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libvoxelproj_hip.so")
 
-EXPORTS = ("vp_forward", "vp_backward", "vp_last_error", "vp_abi_version")
+EXPORTS = ("vp_forward", "vp_backward", "vp_forward_f64", "vp_backward_f64", "vp_last_error",
+           "vp_abi_version")
 ABI_VERSION = 1
 
 _lib = None
@@ -51,7 +52,7 @@ def load():
         i64 = ctypes.c_int64
         sig = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                i64, i64, i64, i64, ctypes.c_void_p]
-        for name in ("vp_forward", "vp_backward"):
+        for name in ("vp_forward", "vp_backward", "vp_forward_f64", "vp_backward_f64"):
             f = getattr(lib, name)
             f.argtypes = sig
             f.restype = ctypes.c_int

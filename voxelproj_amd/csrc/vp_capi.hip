@@ -124,4 +124,30 @@ int vp_backward(int z_order, const float* y, float* x, const float* angles, int 
     return VP_OK;
 }
 
+int vp_forward_f64(int z_order, const double* x, double* y, const float* angles, int n_angles,
+                   int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
+{
+    g_last_error.clear();
+    vp::Geom g;
+    bool empty = false;
+    int rc = validate(z_order, x, y, angles, n_angles, SY, SX, H, PX, &g, &empty);
+    if (rc != VP_OK || empty) return rc;
+    hipError_t e = vp::launch_forward_f64(z_order, x, y, angles, g, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "forward f64 launch");
+    return VP_OK;
+}
+
+int vp_backward_f64(int z_order, const double* y, double* x, const float* angles, int n_angles,
+                    int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
+{
+    g_last_error.clear();
+    vp::Geom g;
+    bool empty = false;
+    int rc = validate(z_order, y, x, angles, n_angles, SY, SX, H, PX, &g, &empty);
+    if (rc != VP_OK || empty) return rc;
+    hipError_t e = vp::launch_backward_f64(z_order, y, x, angles, g, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "backward f64 launch");
+    return VP_OK;
+}
+
 }  // extern "C"

@@ -114,16 +114,25 @@ __global__ __launch_bounds__(64 * NW) void forward_z0_kernel(
     bool folded = false;
     auto fold = [&]() {
         if (a < A) {
+            const int hh0 = hb + lane * HL;
+            const bool full = vec4 && HL == 4 && hh0 + 3 < H;
 #pragma unroll
-            for (int j = 0; j < HL; ++j) {
-                const int hh = hb + lane * HL + j;
-                if (hh < H) {
+            for (int i = 0; i < P; ++i) {
+                const int p = p0 + i;
+                if (p < PX) {
+                    float* py = y + ((int64_t)a * PX + p) * H + hh0;
+                    if (full) {
+                        // one 16-byte read-modify-write per lane: whole cache lines
+                        float4 v = make_float4(part[i][0], part[i][1], part[i][2], part[i][3]);
+                        if (accumulate || folded) {
+                            const float4 o = *reinterpret_cast<const float4*>(py);
+                            v.x += o.x; v.y += o.y; v.z += o.z; v.w += o.w;
+                        }
+                        *reinterpret_cast<float4*>(py) = v;
+                    } else {
 #pragma unroll
-                    for (int i = 0; i < P; ++i) {
-                        const int p = p0 + i;
-                        if (p < PX) {
-                            float* py = y + ((int64_t)a * PX + p) * H + hh;
-                            *py = (accumulate || folded) ? (*py + part[i][j]) : part[i][j];
+                        for (int j = 0; j < HL; ++j) {
+                            if (hh0 + j < H) py[j] = (accumulate || folded) ? (py[j] + part[i][j]) : part[i][j];
                         }
                     }
                 }

@@ -20,6 +20,12 @@ hipError_t launch_backward_z0(const float* y, float* x, const float* angles, con
 hipError_t launch_forward_z0(const float* x, float* y, const float* angles, const Geom& g,
                              int accumulate, hipStream_t s);
 
+// FP64 operator pair, either layout (outputs accumulate).
+hipError_t launch_forward_f64(int z_order, const double* x, double* y, const float* angles,
+                              const Geom& g, hipStream_t s);
+hipError_t launch_backward_f64(int z_order, const double* y, double* x, const float* angles,
+                               const Geom& g, hipStream_t s);
+
 // out[c][r] (+)= in[r][c] for an R x C row-major matrix.
 hipError_t launch_transpose(const float* in, float* out, int64_t R, int64_t C, int accumulate,
                             hipStream_t s);
