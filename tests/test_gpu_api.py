@@ -152,7 +152,7 @@ def test_non_contiguous_device_output_is_written_back():
     x = torch.randn(x_shape[0], device="cuda")
     y_big = torch.zeros((PX, n_angles, H), device="cuda")
     y_view = y_big.transpose(0, 1)  # (n_angles, PX, H), non-contiguous
-    ref = voxelproj.forward(x, angles, None)
+    ref = voxelproj.forward(x, angles, (n_angles, PX, H))
     out = voxelproj.forward(x, angles, y_view)
     assert out is y_view
     torch.testing.assert_close(y_view, ref)
