@@ -124,7 +124,7 @@ def test_values_and_accumulation(kind, z_order):
     def unbox(a):
         return a if isinstance(a, np.ndarray) else a.cpu().numpy()
 
-    out = unbox(voxelproj.forward(box(x), angles, None, z_order=z_order))
+    out = unbox(voxelproj.forward(box(x), angles, Ax.shape, z_order=z_order))
     np.testing.assert_allclose(out, Ax, rtol=0, atol=2e-6 * np.abs(Ax).max())
     acc = unbox(voxelproj.forward(box(x), angles, box(prior_y), z_order=z_order))
     np.testing.assert_allclose(acc, prior_y + Ax, rtol=0, atol=4e-6 * np.abs(prior_y + Ax).max())
