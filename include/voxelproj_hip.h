@@ -65,6 +65,42 @@ VP_API int vp_forward_f64(int z_order, const double *x, double *y, const float *
 VP_API int vp_backward_f64(int z_order, const double *y, double *x, const float *angles, int n_angles,
                            int64_t SY, int64_t SX, int64_t H, int64_t PX, void *stream);
 
+/*
+ * Batched form: many projector instances of uneven size and angle count in
+ * one persistent launch (z_order 0 layout).  No counterpart in the
+ * reference, which projects one volume per call.
+ *
+ * Each instance is described by a vp_batch_item.  The caller passes the
+ * descriptor array twice: `items_host` (host memory, used for validation and
+ * to size the launch) and `items_dev` (an identical copy in device memory,
+ * read by the kernel).  `variant` is the number of H slices per lane of the
+ * kernel configuration (forward: 1, 2 or 4; backward: 1 or 2); see
+ * vp_default_variant.  `grid` is the number of persistent workgroups
+ * (<= 0: one per work item).  Results do not depend on the grid or on the
+ * other instances in the batch.  Because the launch reads only device
+ * memory, it can be captured into a HIP graph and replayed.
+ */
+typedef struct vp_batch_item {
+    const float *in;      /* A: volume x (SY, SX, H); AT: sinogram y (A, PX, H) */
+    float *out;           /* A: sinogram y;           AT: volume x            */
+    const float *angles;  /* n_angles float32 radians, device memory          */
+    int32_t n_angles, SY, SX, H, PX;
+    int32_t accumulate;   /* 1: out += op(in); 0: out = op(in)                */
+} vp_batch_item;
+
+VP_API int vp_forward_batched(int variant, const vp_batch_item *items_host,
+                              const vp_batch_item *items_dev, int n, int grid, void *stream);
+VP_API int vp_backward_batched(int variant, const vp_batch_item *items_host,
+                               const vp_batch_item *items_dev, int n, int grid, void *stream);
+
+/* Kernel variant used by vp_forward (op 0) / vp_backward (op 1) for H slices. */
+VP_API int vp_default_variant(int op, int64_t H);
+
+/* Number of work items (workgroups) one instance contributes to a batched
+ * launch of `op` (0 = A, 1 = AT) under `variant`; < 0 on invalid input. */
+VP_API int64_t vp_batch_blocks(int op, int variant, int n_angles, int64_t SY, int64_t SX,
+                               int64_t H, int64_t PX);
+
 /* Description of the last failure on the calling thread ("" if none). */
 VP_API const char *vp_last_error(void);
 

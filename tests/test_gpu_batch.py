@@ -1,0 +1,123 @@
+"""Batched persistent launches, autotuner and graph capture on the GPU."""
+import json
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as vpo
+from voxelproj_amd import batch, ops
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+# (SY, SX, H, n_angles, PX): uneven sizes, ragged H, tiny and truncated cases
+SHAPES = [(40, 36, 70, 45, 52), (96, 96, 32, 60, 96), (17, 23, 5, 13, 30), (64, 64, 128, 90, 64),
+          (128, 128, 64, 180, 128), (33, 20, 9, 600, 25), (1, 1, 3, 4, 2), (50, 70, 256, 33, 70)]
+
+
+def make_instances(seed=0):
+    g = torch.Generator(device=DEV).manual_seed(seed)
+    rng = np.random.default_rng(seed)
+    xs, ys, angs = [], [], []
+    for SY, SX, H, A, PX in SHAPES:
+        xs.append(torch.randn((SY, SX, H), device=DEV, generator=g))
+        ys.append(torch.randn((A, PX, H), device=DEV, generator=g))
+        angs.append(torch.tensor(rng.uniform(0, np.pi, A).astype(np.float32), device=DEV))
+    return xs, ys, angs
+
+
+def rel(a, b):
+    return float((a.double() - b.double()).norm() / b.double().norm())
+
+
+@pytest.mark.parametrize("variant", batch.FORWARD_VARIANTS)
+def test_forward_batch_matches_single_and_oracle(variant):
+    xs, _, angs = make_instances(1)
+    outs = [torch.full((a.numel(), s[4], s[2]), 7.0, device=DEV) for a, s in zip(angs, SHAPES)]
+    batch.BatchPlan("A", xs, outs, angs, accumulate=False, variant=variant).launch()
+    for i, (x, a, s) in enumerate(zip(xs, angs, SHAPES)):
+        ref = ops.project(x, a, s[4])
+        assert rel(outs[i], ref) < 1e-6, i
+        # batch of one with the same variant: bitwise identical
+        solo = torch.empty_like(outs[i])
+        batch.BatchPlan("A", [x], [solo], [a], accumulate=False, variant=variant).launch()
+        assert torch.equal(solo, outs[i]), i
+        if s[0] * s[1] * s[2] * s[3] < 2e6:
+            yo = vpo.forward_f64(x.cpu().numpy(), a.cpu().numpy(), s[4], 0)
+            assert float(np.linalg.norm(outs[i].cpu().numpy() - yo) / np.linalg.norm(yo)) < 1e-6
+
+
+@pytest.mark.parametrize("variant", batch.BACKWARD_VARIANTS)
+def test_backward_batch_matches_single_and_oracle(variant):
+    xs, ys, angs = make_instances(2)
+    outs = [torch.zeros_like(x) for x in xs]
+    batch.BatchPlan("AT", ys, outs, angs, accumulate=True, variant=variant).launch()
+    for i, (y, a, s) in enumerate(zip(ys, angs, SHAPES)):
+        ref = ops.backproject(y, a, s[0], s[1])
+        assert rel(outs[i], ref) < 1e-6, i
+        solo = torch.zeros_like(outs[i])
+        batch.BatchPlan("AT", [y], [solo], [a], accumulate=True, variant=variant).launch()
+        assert torch.equal(solo, outs[i]), i
+        if s[0] * s[1] * s[2] * s[3] < 2e6:
+            xo = vpo.backward_f64(y.cpu().numpy(), a.cpu().numpy(), s[0], s[1], 0)
+            assert float(np.linalg.norm(outs[i].cpu().numpy() - xo) / np.linalg.norm(xo)) < 1e-6
+
+
+def test_grid_size_does_not_change_results():
+    xs, ys, angs = make_instances(3)
+    res = []
+    for grid in (1, 7, None):
+        outs = [torch.empty((a.numel(), s[4], s[2]), device=DEV) for a, s in zip(angs, SHAPES)]
+        batch.BatchPlan("A", xs, outs, angs, grid=grid).launch()
+        res.append(outs)
+    for r in res[1:]:
+        assert all(torch.equal(a, b) for a, b in zip(res[0], r))
+
+
+def test_autotune_picks_fastest_and_caches(tmp_path, monkeypatch):
+    monkeypatch.setenv("VOXELPROJ_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    batch._cache.clear()
+    xs, ys, angs = make_instances(4)
+    outs = [torch.empty((a.numel(), s[4], s[2]), device=DEV) for a, s in zip(angs, SHAPES)]
+    plan = batch.BatchPlan("A", xs, outs, angs)
+    times = batch.autotune(plan, reps=2)
+    assert set(times) == set(batch.FORWARD_VARIANTS)
+    assert plan.variant == min(times, key=times.get)
+    data = json.loads((tmp_path / "tune.json").read_text())
+    assert any(v["best"] == plan.variant for v in data.values())
+    batch._cache.clear()
+
+
+def test_graph_captured_pair_replays_exactly():
+    xs, _, angs = make_instances(5)
+    ys = [torch.empty((a.numel(), s[4], s[2]), device=DEV) for a, s in zip(angs, SHAPES)]
+    xb = [torch.empty_like(x) for x in xs]
+    pa = batch.BatchPlan("A", xs, ys, angs, accumulate=False)
+    pb = batch.BatchPlan("AT", ys, xb, angs, accumulate=False)
+    pa.launch()
+    pb.launch()
+    torch.cuda.synchronize()
+    ref_y = [t.clone() for t in ys]
+    ref_x = [t.clone() for t in xb]
+    graph = batch.capture_pair(pa, pb)
+    for rep in range(3):
+        for t in ys + xb:
+            t.fill_(float("nan"))           # eager work between replays
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert all(torch.equal(a, b) for a, b in zip(ys, ref_y)), rep
+        assert all(torch.equal(a, b) for a, b in zip(xb, ref_x)), rep
+
+
+def test_plan_validation():
+    x = torch.zeros((8, 8, 4), device=DEV)
+    a = torch.zeros(5, device=DEV)
+    with pytest.raises(ValueError):
+        batch.BatchPlan("A", [x], [torch.zeros((6, 8, 4), device=DEV)], [a])
+    with pytest.raises(ValueError):
+        batch.BatchPlan("A", [x], [torch.zeros((5, 8, 3), device=DEV)], [a])
+    plan = batch.BatchPlan("A", [x], [torch.zeros((5, 8, 4), device=DEV)], [a])
+    with pytest.raises(RuntimeError, match="variant"):
+        plan.launch(variant=3)

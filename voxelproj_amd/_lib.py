@@ -12,8 +12,9 @@ import torch  # noqa: F401  -- loads torch's HIP runtime first, so the library b
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libvoxelproj_hip.so")
 
-EXPORTS = ("vp_forward", "vp_backward", "vp_forward_f64", "vp_backward_f64", "vp_last_error",
-           "vp_abi_version")
+EXPORTS = ("vp_forward", "vp_backward", "vp_forward_f64", "vp_backward_f64",
+           "vp_forward_batched", "vp_backward_batched", "vp_default_variant", "vp_batch_blocks",
+           "vp_last_error", "vp_abi_version")
 ABI_VERSION = 1
 
 _lib = None
@@ -56,6 +57,15 @@ def load():
             f = getattr(lib, name)
             f.argtypes = sig
             f.restype = ctypes.c_int
+        for name in ("vp_forward_batched", "vp_backward_batched"):
+            f = getattr(lib, name)
+            f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                          ctypes.c_int, ctypes.c_void_p]
+            f.restype = ctypes.c_int
+        lib.vp_default_variant.argtypes = [ctypes.c_int, i64]
+        lib.vp_default_variant.restype = ctypes.c_int
+        lib.vp_batch_blocks.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, i64, i64, i64, i64]
+        lib.vp_batch_blocks.restype = i64
         lib.vp_last_error.argtypes = []
         lib.vp_last_error.restype = ctypes.c_char_p
         lib.vp_abi_version.argtypes = []

@@ -1,0 +1,262 @@
+"""Batched projectors: many volumes of uneven size and angle count per launch.
+
+A ``BatchPlan`` holds the descriptors of a list of z_order-0 instances (host
+copy for validation, device copy for the kernel) and launches them as one
+persistent grouped kernel through ``vp_forward_batched`` /
+``vp_backward_batched``.  Plans only read device memory when launched, so an
+A plan followed by an AT plan can be captured into a HIP graph
+(``capture_pair``) and replayed.
+
+The kernel variant (H slices per lane, which fixes the tile shapes) is
+chosen by ``autotune``: it times every variant on the actual batch and
+caches the winner per batch signature, in memory and in a JSON file
+(``$VOXELPROJ_AUTOTUNE_CACHE``, default ``~/.cache/voxelproj_amd/autotune.json``).
+
+Every instance's result is independent of the others in the batch and of
+the grid size (fixed work order, no atomics): a batch of one and a batch of
+many give bitwise identical results for the same variant.
+
+No counterpart in the reference, which projects one volume per call
+(voxelproj/wrappers.py:28-122).
+"""
+import ctypes
+import json
+import math
+import os
+import threading
+
+import numpy as np
+import torch
+
+from . import _lib, ops, recon
+
+FORWARD_VARIANTS = (1, 2, 4)
+BACKWARD_VARIANTS = (1, 2)
+
+
+class _Item(ctypes.Structure):
+    _fields_ = [
+        ("in_", ctypes.c_void_p),
+        ("out", ctypes.c_void_p),
+        ("angles", ctypes.c_void_p),
+        ("n_angles", ctypes.c_int32),
+        ("SY", ctypes.c_int32),
+        ("SX", ctypes.c_int32),
+        ("H", ctypes.c_int32),
+        ("PX", ctypes.c_int32),
+        ("accumulate", ctypes.c_int32),
+    ]
+
+
+def default_grid(device=None):
+    """Persistent workgroups: two per compute unit."""
+    props = torch.cuda.get_device_properties(device if device is not None else torch.cuda.current_device())
+    return 2 * props.multi_processor_count
+
+
+class BatchPlan:
+    """One batched launch of A (op='A') or AT (op='AT') over z_order-0 instances.
+
+    ins / outs: lists of contiguous float32 device tensors; angles: list of
+    float32 device tensors.  For A, ins are volumes (SY, SX, H) and outs
+    sinograms (A, PX, H); for AT the other way round.  accumulate=False
+    overwrites the outputs.
+    """
+
+    def __init__(self, op, ins, outs, angles, accumulate=False, variant=None, grid=None):
+        if op not in ("A", "AT"):
+            raise ValueError("op must be 'A' or 'AT'")
+        if not (len(ins) == len(outs) == len(angles)) or len(ins) == 0:
+            raise ValueError("ins, outs and angles must be non-empty lists of equal length")
+        self.op = op
+        self.device = ins[0].device
+        self.ins, self.outs = list(ins), list(outs)
+        self.angles = [ops.device_angles(a, self.device) for a in angles]
+        n = len(ins)
+        arr = (_Item * n)()
+        self.shapes = []
+        for i, (a, b, ang) in enumerate(zip(self.ins, self.outs, self.angles)):
+            for t, nm in ((a, "input"), (b, "output")):
+                if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and t.dim() == 3):
+                    raise ValueError(f"instance {i}: {nm} must be a contiguous 3-D float32 device tensor")
+                if t.device != self.device:
+                    raise ValueError(f"instance {i}: all tensors must be on {self.device}")
+            if op == "A":
+                SY, SX, H = a.shape
+                A_, PX, H2 = b.shape
+            else:
+                A_, PX, H = a.shape
+                SY, SX, H2 = b.shape
+            if H2 != H or A_ != ang.numel():
+                raise ValueError(f"instance {i}: inconsistent shapes {tuple(a.shape)} / {tuple(b.shape)} "
+                                 f"with {ang.numel()} angles")
+            arr[i] = _Item(a.data_ptr(), b.data_ptr(), ang.data_ptr(), A_, SY, SX, H, PX,
+                           1 if accumulate else 0)
+            self.shapes.append((int(SY), int(SX), int(H), int(A_), int(PX)))
+        self._host = arr
+        raw = np.frombuffer(bytes(arr), dtype=np.uint8)
+        self._dev = torch.tensor(raw, dtype=torch.uint8, device=self.device)
+        self.n = n
+        self.grid = default_grid(self.device) if grid is None else int(grid)
+        self.variant = variant if variant is not None else _default_variant(op, self.shapes)
+
+    @property
+    def work(self):
+        """voxel-angles of the batch (SY*SX*H*A summed over instances)."""
+        return sum(SY * SX * H * A for SY, SX, H, A, _ in self.shapes)
+
+    def signature(self):
+        return f"{self.op}:" + ";".join(",".join(map(str, s)) for s in sorted(self.shapes))
+
+    def launch(self, variant=None):
+        lib = _lib.load()
+        v = self.variant if variant is None else variant
+        fn = lib.vp_forward_batched if self.op == "A" else lib.vp_backward_batched
+        with torch.cuda.device(self.device):
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+            rc = fn(int(v), ctypes.addressof(self._host), self._dev.data_ptr(), self.n,
+                    self.grid, stream)
+        _lib.check(rc, "batched " + self.op)
+        return self.outs
+
+
+def _default_variant(op, shapes):
+    lib = _lib.load()
+    hs = sorted(s[2] for s in shapes)
+    return int(lib.vp_default_variant(0 if op == "A" else 1, hs[len(hs) // 2]))
+
+
+# --------------------------------------------------------------- autotuner
+_cache = {}
+_cache_lock = threading.Lock()
+
+
+def _cache_path():
+    return os.environ.get("VOXELPROJ_AUTOTUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "voxelproj_amd",
+                                       "autotune.json"))
+
+
+def _load_cache():
+    p = _cache_path()
+    try:
+        with open(p) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store_cache(entries):
+    p = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(p), exist_ok=True)
+        tmp = p + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(entries, f, indent=1, sort_keys=True)
+        os.replace(tmp, p)
+    except OSError:
+        pass  # a read-only home only costs re-tuning
+
+
+def autotune(plan, reps=3, persist=True):
+    """Time every kernel variant on `plan` (outputs are overwritten or
+    accumulated into as the plan says, so use scratch outputs), pick the
+    fastest, store it in the plan and the cache, and return {variant: ms}."""
+    key = f"{torch.cuda.get_device_name(plan.device)}|{plan.signature()}|grid={plan.grid}"
+    with _cache_lock:
+        if not _cache:
+            _cache.update(_load_cache())
+        if key in _cache:
+            plan.variant = int(_cache[key]["best"])
+            return {int(k): v for k, v in _cache[key]["ms"].items()}
+    variants = FORWARD_VARIANTS if plan.op == "A" else BACKWARD_VARIANTS
+    times = {}
+    stream = torch.cuda.current_stream(plan.device)
+    for v in variants:
+        plan.launch(v)  # warm-up (code object load, caches)
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(reps):
+            plan.launch(v)
+        end.record(stream)
+        end.synchronize()
+        times[v] = start.elapsed_time(end) / reps
+    best = min(times, key=times.get)
+    plan.variant = best
+    with _cache_lock:
+        _cache[key] = {"best": best, "ms": {str(k): t for k, t in times.items()}}
+        if persist:
+            _store_cache(dict(_cache))
+    return times
+
+
+# --------------------------------------------------------------- graphs
+def capture_pair(plan_a, plan_at, warmup=1):
+    """Capture `plan_a.launch()` followed by `plan_at.launch()` into a graph.
+
+    Returns a torch.cuda.CUDAGraph; replay() re-runs both launches on the
+    same buffers.  Use accumulate=False plans (no fill kernels needed)."""
+    s = torch.cuda.Stream(device=plan_a.device)
+    s.wait_stream(torch.cuda.current_stream(plan_a.device))
+    with torch.cuda.stream(s):
+        for _ in range(warmup):
+            plan_a.launch()
+            plan_at.launch()
+    torch.cuda.current_stream(plan_a.device).wait_stream(s)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        plan_a.launch()
+        plan_at.launch()
+    return g
+
+
+# --------------------------------------------------------------- workload
+def uneven_workload(n=48, seed=0, device="cuda"):
+    """The batched benchmark set: n volumes with SX = SY from
+    {96, 128, 192, 256, 384}, H from {32, 64, 128, 256}, n_angles from
+    {60, 180, 360, 720, 1024}, angles uniform in [0, pi) with +-0.5 degree
+    seeded jitter; half N(0,1) volumes, half ellipsoid phantoms.
+
+    Returns a list of (x, angles) device tensors (x in z_order 0)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        S = int(rng.choice([96, 128, 192, 256, 384]))
+        H = int(rng.choice([32, 64, 128, 256]))
+        A = int(rng.choice([60, 180, 360, 720, 1024]))
+        ang = np.linspace(0, np.pi, A, endpoint=False) + rng.uniform(-0.5, 0.5, A) * math.pi / 180
+        ang_t = torch.tensor(ang.astype(np.float32), device=device)
+        if i % 2 == 0:
+            g = torch.Generator(device=device).manual_seed(seed * 1000 + i)
+            x = torch.randn((S, S, H), device=device, generator=g)
+        else:
+            x = recon.ellipsoid_phantom(S, S, H, seed=seed * 1000 + i, device=device)
+        out.append((x, ang_t))
+    return out
+
+
+def forward_batch(xs, angles, ys=None, PX=None, accumulate=False, variant=None, grid=None):
+    """A for a list of volumes in one launch; returns the list of sinograms."""
+    if ys is None:
+        ys = []
+        for x, a in zip(xs, angles):
+            SY, SX, H = x.shape
+            px = max(SY, SX) if PX is None else PX
+            ys.append(torch.empty((ops.device_angles(a, x.device).numel(), px, H),
+                                  device=x.device, dtype=torch.float32))
+        accumulate = False
+    BatchPlan("A", xs, ys, angles, accumulate, variant, grid).launch()
+    return ys
+
+
+def backward_batch(ys, angles, xs=None, accumulate=False, variant=None, grid=None):
+    """AT for a list of sinograms in one launch; returns the list of volumes."""
+    if xs is None:
+        xs = []
+        for y, a in zip(ys, angles):
+            A_, PX, H = y.shape
+            xs.append(torch.empty((PX, PX, H), device=y.device, dtype=torch.float32))
+        accumulate = False
+    BatchPlan("AT", ys, xs, angles, accumulate, variant, grid).launch()
+    return xs

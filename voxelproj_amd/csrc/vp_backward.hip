@@ -21,6 +21,8 @@
 #include "vp_device.hpp"
 #include "vp_launch.hpp"
 
+#include <algorithm>
+
 namespace vp {
 namespace {
 
@@ -33,12 +35,26 @@ __device__ __forceinline__ float rdlane(float v, int l)
 __device__ __forceinline__ int rdlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-template <int TX, int TY, int NW, int AC, int WIN, int HL>
-__global__ __launch_bounds__(64 * NW) void backproject_z0_kernel(
-    const float* __restrict__ y, float* __restrict__ x, const float* __restrict__ angles,
-    int A, int SY, int SX, int H, int PX, double scale, double inv_scale, int accumulate,
-    int nbx, int nby)
+template <int TX, int TY, int HL>
+__host__ __device__ inline void backward_grid(int64_t SY, int64_t SX, int64_t H, int& nbx, int& nby,
+                                              int& nbh)
 {
+    nbx = (int)((SX + TX - 1) / TX);
+    nby = (int)((SY + TY - 1) / TY);
+    nbh = (int)((H + 64 * HL - 1) / (64 * HL));
+}
+
+template <int TX, int TY, int NW, int AC, int WIN, int HL>
+__device__ __forceinline__ void backproject_block(const OpArgs& g, const int logical)
+{
+    const float* __restrict__ y = g.in;
+    float* __restrict__ x = g.out;
+    const float* __restrict__ angles = g.angles;
+    const int A = g.A, SY = g.SY, SX = g.SX, H = g.H, PX = g.PX;
+    const double scale = g.scale, inv_scale = g.inv_scale;
+    const int accumulate = g.accumulate;
+    int nbx, nby, nbh_;
+    backward_grid<TX, TY, HL>(SY, SX, H, nbx, nby, nbh_);
     constexpr int VPW = TX * TY / NW;        // voxels per wave
     constexpr int APL = 64 / VPW;            // angles evaluated per lane sweep
     constexpr int VOX = 64 * HL;             // floats per staged pixel row
@@ -54,12 +70,6 @@ __global__ __launch_bounds__(64 * NW) void backproject_z0_kernel(
     const int lane = tid & 63;
     const int wave = uni(tid >> 6);
 
-    // XCD-aware order: XCD k runs a contiguous range of logical tiles, so
-    // neighbouring tiles (which share sinogram rows) meet in the same L2.
-    const int nb = gridDim.x;
-    const int bid = blockIdx.x;
-    const int per = nb / kXcds;
-    const int logical = (bid < per * kXcds) ? (bid % kXcds) * per + bid / kXcds : bid;
     const int bx = logical % nbx;
     const int by = (logical / nbx) % nby;
     const int bhc = logical / (nbx * nby);
@@ -231,23 +241,140 @@ __global__ __launch_bounds__(64 * NW) void backproject_z0_kernel(
     }
 }
 
+// XCD-aware order: XCD k runs a contiguous range of logical tiles, so
+// neighbouring tiles (which share sinogram rows) meet in the same L2.
+__device__ __forceinline__ int xcd_logical(int bid, int nb)
+{
+    const int per = nb / kXcds;
+    return (bid < per * kXcds) ? (bid % kXcds) * per + bid / kXcds : bid;
+}
+
+__device__ __forceinline__ OpArgs item_args(const BatchItem& it)
+{
+    OpArgs g;
+    g.in = it.in;
+    g.out = it.out;
+    g.angles = it.angles;
+    g.A = it.A;
+    g.SY = it.SY;
+    g.SX = it.SX;
+    g.H = it.H;
+    g.PX = it.PX;
+    g.accumulate = it.accumulate;
+    const int bits = hi_bits(it.SY, it.SX, it.PX);
+    g.scale = ldexp(1.0, bits);
+    g.inv_scale = ldexp(1.0, -bits);
+    return g;
+}
+
+template <int TX, int TY, int NW, int AC, int WIN, int HL>
+__global__ __launch_bounds__(64 * NW) void backproject_z0_kernel(OpArgs g)
+{
+    backproject_block<TX, TY, NW, AC, WIN, HL>(g, xcd_logical(blockIdx.x, gridDim.x));
+}
+
+// Persistent launch over a list of instances (fixed work order, see the
+// forward kernel of the same name).
+template <int TX, int TY, int NW, int AC, int WIN, int HL>
+__global__ __launch_bounds__(64 * NW) void backproject_batched_kernel(
+    const BatchItem* __restrict__ items, int n, int64_t total)
+{
+    for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
+        int i = 0;
+        int64_t base = 0;
+        for (; i < n - 1; ++i) {
+            int nbx, nby, nbh;
+            backward_grid<TX, TY, HL>(items[i].SY, items[i].SX, items[i].H, nbx, nby, nbh);
+            const int64_t nbl = (int64_t)nbx * nby * nbh;
+            if (w < base + nbl) break;
+            base += nbl;
+        }
+        backproject_block<TX, TY, NW, AC, WIN, HL>(item_args(items[i]), (int)(w - base));
+        __syncthreads();
+    }
+}
+
+template <int HL>
+struct BwdCfg {
+    static constexpr int TX = 16, TY = 16, NW = 8, AC = 4, WIN = 28;
+};
+
+template <int HL>
+hipError_t backward_single(const OpArgs& g, hipStream_t s)
+{
+    using C = BwdCfg<HL>;
+    int nbx, nby, nbh;
+    backward_grid<C::TX, C::TY, HL>(g.SY, g.SX, g.H, nbx, nby, nbh);
+    const int64_t nb = (int64_t)nbx * nby * nbh;
+    if (nb > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((backproject_z0_kernel<C::TX, C::TY, C::NW, C::AC, C::WIN, HL>),
+                       dim3((unsigned)nb), dim3(64 * C::NW), 0, s, g);
+    return hipGetLastError();
+}
+
+template <int HL>
+hipError_t backward_batched(const BatchItem* host, const BatchItem* dev, int n, int grid,
+                            hipStream_t s)
+{
+    using C = BwdCfg<HL>;
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        int nbx, nby, nbh;
+        backward_grid<C::TX, C::TY, HL>(host[i].SY, host[i].SX, host[i].H, nbx, nby, nbh);
+        total += (int64_t)nbx * nby * nbh;
+    }
+    if (total == 0) return hipSuccess;
+    const int64_t gsz = grid > 0 ? std::min<int64_t>(grid, total) : total;
+    hipLaunchKernelGGL((backproject_batched_kernel<C::TX, C::TY, C::NW, C::AC, C::WIN, HL>),
+                       dim3((unsigned)gsz), dim3(64 * C::NW), 0, s, dev, n, total);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+int default_variant(int op, int64_t H)
+{
+    if (op == 0) return H > 128 ? 4 : (H > 64 ? 2 : 1);
+    return H > 64 ? 2 : 1;
+}
 
 hipError_t launch_backward_z0(const float* y, float* x, const float* angles, const Geom& g,
                               int accumulate, hipStream_t s)
 {
-    constexpr int TX = 16, TY = 16, NW = 8, AC = 4, WIN = 28, HL = 2;
-    const double scale = ldexp(1.0, g.bits);
-    const double inv_scale = ldexp(1.0, -g.bits);
-    const int nbx = (int)((g.SX + TX - 1) / TX);
-    const int nby = (int)((g.SY + TY - 1) / TY);
-    const int nbh = (int)((g.H + 64 * HL - 1) / (64 * HL));
-    const int64_t nb = (int64_t)nbx * nby * nbh;
-    if (nb > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((backproject_z0_kernel<TX, TY, NW, AC, WIN, HL>), dim3((unsigned)nb),
-                       dim3(64 * NW), 0, s, y, x, angles, g.A, (int)g.SY, (int)g.SX, (int)g.H,
-                       (int)g.PX, scale, inv_scale, accumulate, nbx, nby);
-    return hipGetLastError();
+    OpArgs a;
+    a.in = y;
+    a.out = x;
+    a.angles = angles;
+    a.A = g.A;
+    a.SY = (int)g.SY;
+    a.SX = (int)g.SX;
+    a.H = (int)g.H;
+    a.PX = (int)g.PX;
+    a.accumulate = accumulate;
+    a.scale = ldexp(1.0, g.bits);
+    a.inv_scale = ldexp(1.0, -g.bits);
+    return default_variant(1, g.H) == 1 ? backward_single<1>(a, s) : backward_single<2>(a, s);
+}
+
+hipError_t launch_backward_batched(int variant, const BatchItem* items_host,
+                                   const BatchItem* items_dev, int n, int grid, hipStream_t s)
+{
+    switch (variant) {
+        case 1: return backward_batched<1>(items_host, items_dev, n, grid, s);
+        case 2: return backward_batched<2>(items_host, items_dev, n, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+int64_t backward_blocks(int variant, int64_t SY, int64_t SX, int64_t H)
+{
+    int nbx = 0, nby = 0, nbh = 0;
+    switch (variant) {
+        case 1: backward_grid<16, 16, 1>(SY, SX, H, nbx, nby, nbh); break;
+        case 2: backward_grid<16, 16, 2>(SY, SX, H, nbx, nby, nbh); break;
+        default: return -1;
+    }
+    return (int64_t)nbx * nby * nbh;
 }
 
 }  // namespace vp

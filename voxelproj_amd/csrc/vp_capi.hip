@@ -1,6 +1,7 @@
 // C ABI of the projector library (declared in include/voxelproj_hip.h).
 // Validation happens here, before anything touches the device; kernels are
 // launched stream-ordered on the caller's stream and every launch is checked.
+#include <stddef.h>
 #include <stdio.h>
 #include <string>
 
@@ -47,6 +48,32 @@ int validate(int z_order, const void* in, const void* out, const float* angles, 
     g->PX = PX;
     g->A = n_angles;
     g->bits = vp::hi_bits(SY, SX, PX);
+    return VP_OK;
+}
+
+static_assert(sizeof(vp_batch_item) == sizeof(vp::BatchItem), "batch item layout");
+static_assert(offsetof(vp_batch_item, n_angles) == offsetof(vp::BatchItem, A), "batch item layout");
+static_assert(offsetof(vp_batch_item, accumulate) == offsetof(vp::BatchItem, accumulate),
+              "batch item layout");
+
+int validate_batch(int op, int variant, const vp_batch_item* items, int n)
+{
+    if (n < 0) return fail(VP_ERR_ARG, "negative batch size");
+    if (n > 0 && items == nullptr) return fail(VP_ERR_ARG, "null descriptor array");
+    if ((op == 0 ? vp::forward_blocks(variant, 1, 1, 1) : vp::backward_blocks(variant, 1, 1, 1)) < 0)
+        return fail(VP_ERR_ARG, "unknown kernel variant " + std::to_string(variant));
+    for (int i = 0; i < n; ++i) {
+        const vp_batch_item& it = items[i];
+        vp::Geom g;
+        bool empty = false;
+        const void* in = it.in;
+        const void* out = it.out;
+        int rc = validate(0, in, out, it.angles, it.n_angles, it.SY, it.SX, it.H, it.PX, &g, &empty);
+        if (rc != VP_OK) return fail(rc, "batch item " + std::to_string(i) + ": " + g_last_error);
+        if (empty) return fail(VP_ERR_ARG, "batch item " + std::to_string(i) + " is empty");
+        if (it.accumulate != 0 && it.accumulate != 1)
+            return fail(VP_ERR_ARG, "batch item " + std::to_string(i) + ": accumulate must be 0 or 1");
+    }
     return VP_OK;
 }
 
@@ -147,6 +174,45 @@ int vp_backward_f64(int z_order, const double* y, double* x, const float* angles
     if (rc != VP_OK || empty) return rc;
     hipError_t e = vp::launch_backward_f64(z_order, y, x, angles, g, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "backward f64 launch");
+    return VP_OK;
+}
+
+int vp_default_variant(int op, int64_t H) { return vp::default_variant(op, H); }
+
+int64_t vp_batch_blocks(int op, int variant, int n_angles, int64_t SY, int64_t SX, int64_t H,
+                        int64_t PX)
+{
+    if (n_angles <= 0 || SY <= 0 || SX <= 0 || H <= 0 || PX <= 0) return -1;
+    if (op == 0) return vp::forward_blocks(variant, n_angles, H, PX);
+    if (op == 1) return vp::backward_blocks(variant, SY, SX, H);
+    return -1;
+}
+
+int vp_forward_batched(int variant, const vp_batch_item* items_host, const vp_batch_item* items_dev,
+                       int n, int grid, void* stream)
+{
+    g_last_error.clear();
+    int rc = validate_batch(0, variant, items_host, n);
+    if (rc != VP_OK || n == 0) return rc;
+    if (items_dev == nullptr) return fail(VP_ERR_ARG, "null device descriptor array");
+    hipError_t e = vp::launch_forward_batched(
+        variant, reinterpret_cast<const vp::BatchItem*>(items_host),
+        reinterpret_cast<const vp::BatchItem*>(items_dev), n, grid, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "batched forward launch");
+    return VP_OK;
+}
+
+int vp_backward_batched(int variant, const vp_batch_item* items_host, const vp_batch_item* items_dev,
+                        int n, int grid, void* stream)
+{
+    g_last_error.clear();
+    int rc = validate_batch(1, variant, items_host, n);
+    if (rc != VP_OK || n == 0) return rc;
+    if (items_dev == nullptr) return fail(VP_ERR_ARG, "null device descriptor array");
+    hipError_t e = vp::launch_backward_batched(
+        variant, reinterpret_cast<const vp::BatchItem*>(items_host),
+        reinterpret_cast<const vp::BatchItem*>(items_dev), n, grid, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "batched backward launch");
     return VP_OK;
 }
 

@@ -18,6 +18,8 @@
 #include "vp_device.hpp"
 #include "vp_launch.hpp"
 
+#include <algorithm>
+
 namespace vp {
 namespace {
 
@@ -50,12 +52,25 @@ struct Task {
 // current one is processed (double-buffered LDS, one barrier per piece).
 // Angles whose dominant axis is x and those whose dominant axis is y are
 // handled in two passes over the volume.
-template <int P, int NW, int HL, int UMAX>
-__global__ __launch_bounds__(64 * NW) void forward_z0_kernel(
-    const float* __restrict__ x, float* __restrict__ y, const float* __restrict__ angles,
-    int A, int SY, int SX, int H, int PX, double scale, double inv_scale, int accumulate,
-    int nbp, int nba)
+template <int P, int NW, int HL>
+__host__ __device__ inline void forward_grid(int A, int64_t H, int64_t PX, int& nbp, int& nba, int& nbh)
 {
+    nbp = (int)((PX + P - 1) / P);
+    nba = (A + NW - 1) / NW;
+    nbh = (int)((H + 64 * HL - 1) / (64 * HL));
+}
+
+template <int P, int NW, int HL, int UMAX>
+__device__ __forceinline__ void forward_block(const OpArgs& g, const int logical)
+{
+    const float* __restrict__ x = g.in;
+    float* __restrict__ y = g.out;
+    const float* __restrict__ angles = g.angles;
+    const int A = g.A, SY = g.SY, SX = g.SX, H = g.H, PX = g.PX;
+    const double scale = g.scale, inv_scale = g.inv_scale;
+    const int accumulate = g.accumulate;
+    int nbp, nba, nbh;
+    forward_grid<P, NW, HL>(A, H, PX, nbp, nba, nbh);
     static_assert((P + 5) * 1.4143 + 4 <= kSeg, "segment may exceed one wave");
     constexpr int VOX = 64 * HL;                 // floats per staged voxel
     constexpr int STAGE = UMAX * VOX;            // floats per buffer
@@ -69,10 +84,6 @@ __global__ __launch_bounds__(64 * NW) void forward_z0_kernel(
     const int lane = tid & 63;
     const int wave = uni(tid >> 6);
 
-    const int nb = gridDim.x;
-    const int bid = blockIdx.x;
-    const int per = nb / kXcds;
-    const int logical = (bid < per * kXcds) ? (bid % kXcds) * per + bid / kXcds : bid;
     const int ba = logical % nba;
     const int bp = (logical / nba) % nbp;
     const int bh = logical / (nba * nbp);
@@ -115,21 +126,26 @@ __global__ __launch_bounds__(64 * NW) void forward_z0_kernel(
     auto fold = [&]() {
         if (a < A) {
             const int hh0 = hb + lane * HL;
-            const bool full = vec4 && HL == 4 && hh0 + 3 < H;
+            const bool full = vec4 && hh0 + 3 < H;
 #pragma unroll
             for (int i = 0; i < P; ++i) {
                 const int p = p0 + i;
                 if (p < PX) {
                     float* py = y + ((int64_t)a * PX + p) * H + hh0;
-                    if (full) {
-                        // one 16-byte read-modify-write per lane: whole cache lines
-                        float4 v = make_float4(part[i][0], part[i][1], part[i][2], part[i][3]);
-                        if (accumulate || folded) {
-                            const float4 o = *reinterpret_cast<const float4*>(py);
-                            v.x += o.x; v.y += o.y; v.z += o.z; v.w += o.w;
+                    bool done = false;
+                    if constexpr (HL == 4) {
+                        if (full) {
+                            // one 16-byte read-modify-write per lane: whole cache lines
+                            float4 v = make_float4(part[i][0], part[i][1], part[i][2], part[i][3]);
+                            if (accumulate || folded) {
+                                const float4 o = *reinterpret_cast<const float4*>(py);
+                                v.x += o.x; v.y += o.y; v.z += o.z; v.w += o.w;
+                            }
+                            *reinterpret_cast<float4*>(py) = v;
+                            done = true;
                         }
-                        *reinterpret_cast<float4*>(py) = v;
-                    } else {
+                    }
+                    if (!done) {
 #pragma unroll
                         for (int j = 0; j < HL; ++j) {
                             if (hh0 + j < H) py[j] = (accumulate || folded) ? (py[j] + part[i][j]) : part[i][j];
@@ -214,11 +230,17 @@ __global__ __launch_bounds__(64 * NW) void forward_z0_kernel(
             const int nv = t.j1 - t.j0 + 1;
             const float* base = x + (int64_t)t.row * st_row + (int64_t)t.j0 * st_along;
             if (vec4) {
-                const int hoff = min(hb + lane * 4, H - 4);
-                for (int v = wave; v < nv; v += NW) {
+                constexpr int RPI = 1024 / (VOX * 4);  // voxels per 16-byte-per-lane DMA
+                constexpr int LPR = 64 / RPI;          // lanes per voxel
+                static_assert(UMAX % RPI == 0, "staging buffer must hold whole DMA groups");
+                const int sub = lane / LPR;
+                const int hoff = min(hb + (lane % LPR) * 4, H - 4);
+                const int nins = (nv + RPI - 1) / RPI;
+                for (int ins = wave; ins < nins; ins += NW) {
+                    const int v = min(ins * RPI + sub, nv - 1);
                     __builtin_amdgcn_global_load_lds(
                         (const void*)(base + (int64_t)v * st_along + hoff),
-                        (__attribute__((address_space(3))) void*)&s_x4[b][v * (VOX / 4)], 16, 0, 0);
+                        (__attribute__((address_space(3))) void*)&s_x4[b][ins * RPI * (VOX / 4)], 16, 0, 0);
                 }
             } else {
                 for (int v = wave; v < nv; v += NW) {
@@ -385,6 +407,62 @@ __global__ __launch_bounds__(64 * NW) void forward_z0_kernel(
     fold();
 }
 
+// XCD-aware order: logical blocks [k*nb/8, (k+1)*nb/8) all run on XCD k,
+// and consecutive logical blocks are the angle groups of one pixel range,
+// which read the same volume strip.
+__device__ __forceinline__ int xcd_logical(int bid, int nb)
+{
+    const int per = nb / kXcds;
+    return (bid < per * kXcds) ? (bid % kXcds) * per + bid / kXcds : bid;
+}
+
+__device__ __forceinline__ OpArgs item_args(const BatchItem& it)
+{
+    OpArgs g;
+    g.in = it.in;
+    g.out = it.out;
+    g.angles = it.angles;
+    g.A = it.A;
+    g.SY = it.SY;
+    g.SX = it.SX;
+    g.H = it.H;
+    g.PX = it.PX;
+    g.accumulate = it.accumulate;
+    const int bits = hi_bits(it.SY, it.SX, it.PX);
+    g.scale = ldexp(1.0, bits);
+    g.inv_scale = ldexp(1.0, -bits);
+    return g;
+}
+
+template <int P, int NW, int HL, int UMAX>
+__global__ __launch_bounds__(64 * NW) void forward_z0_kernel(OpArgs g)
+{
+    forward_block<P, NW, HL, UMAX>(g, xcd_logical(blockIdx.x, gridDim.x));
+}
+
+// Persistent launch over a list of instances: block b handles work items
+// b, b + gridDim.x, ... of the concatenated (instance, block) list, in a
+// fixed order, so results do not depend on the grid or on the other
+// instances of the batch.
+template <int P, int NW, int HL, int UMAX>
+__global__ __launch_bounds__(64 * NW) void forward_batched_kernel(const BatchItem* __restrict__ items,
+                                                                  int n, int64_t total)
+{
+    for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
+        int i = 0;
+        int64_t base = 0;
+        for (; i < n - 1; ++i) {
+            int nbp, nba, nbh;
+            forward_grid<P, NW, HL>(items[i].A, items[i].H, items[i].PX, nbp, nba, nbh);
+            const int64_t nbl = (int64_t)nbp * nba * nbh;
+            if (w < base + nbl) break;
+            base += nbl;
+        }
+        forward_block<P, NW, HL, UMAX>(item_args(items[i]), (int)(w - base));
+        __syncthreads();
+    }
+}
+
 constexpr int kTile = 64;
 
 __global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict__ in,
@@ -410,23 +488,88 @@ __global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict_
     }
 }
 
+template <int HL>
+struct FwdCfg;
+template <> struct FwdCfg<1> { static constexpr int P = 32, NW = 8, UMAX = 128; };
+template <> struct FwdCfg<2> { static constexpr int P = 32, NW = 8, UMAX = 128; };
+template <> struct FwdCfg<4> { static constexpr int P = 32, NW = 8, UMAX = 72; };
+
+template <int HL>
+hipError_t forward_single(const OpArgs& g, hipStream_t s)
+{
+    using C = FwdCfg<HL>;
+    int nbp, nba, nbh;
+    forward_grid<C::P, C::NW, HL>(g.A, g.H, g.PX, nbp, nba, nbh);
+    const int64_t nb = (int64_t)nbp * nba * nbh;
+    if (nb > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((forward_z0_kernel<C::P, C::NW, HL, C::UMAX>), dim3((unsigned)nb),
+                       dim3(64 * C::NW), 0, s, g);
+    return hipGetLastError();
+}
+
+template <int HL>
+hipError_t forward_batched(const BatchItem* host, const BatchItem* dev, int n, int grid,
+                           hipStream_t s)
+{
+    using C = FwdCfg<HL>;
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        int nbp, nba, nbh;
+        forward_grid<C::P, C::NW, HL>(host[i].A, host[i].H, host[i].PX, nbp, nba, nbh);
+        total += (int64_t)nbp * nba * nbh;
+    }
+    if (total == 0) return hipSuccess;
+    const int64_t gsz = grid > 0 ? std::min<int64_t>(grid, total) : total;
+    hipLaunchKernelGGL((forward_batched_kernel<C::P, C::NW, HL, C::UMAX>), dim3((unsigned)gsz),
+                       dim3(64 * C::NW), 0, s, dev, n, total);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_forward_z0(const float* x, float* y, const float* angles, const Geom& g,
                              int accumulate, hipStream_t s)
 {
-    constexpr int P = 32, NW = 8, HL = 4, UMAX = 72;
-    const double scale = ldexp(1.0, g.bits);
-    const double inv_scale = ldexp(1.0, -g.bits);
-    const int nbp = (int)((g.PX + P - 1) / P);
-    const int nba = (g.A + NW - 1) / NW;
-    const int nbh = (int)((g.H + 64 * HL - 1) / (64 * HL));
-    const int64_t nb = (int64_t)nbp * nba * nbh;
-    if (nb > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((forward_z0_kernel<P, NW, HL, UMAX>), dim3((unsigned)nb), dim3(64 * NW), 0, s, x,
-                       y, angles, g.A, (int)g.SY, (int)g.SX, (int)g.H, (int)g.PX, scale, inv_scale,
-                       accumulate, nbp, nba);
-    return hipGetLastError();
+    OpArgs a;
+    a.in = x;
+    a.out = y;
+    a.angles = angles;
+    a.A = g.A;
+    a.SY = (int)g.SY;
+    a.SX = (int)g.SX;
+    a.H = (int)g.H;
+    a.PX = (int)g.PX;
+    a.accumulate = accumulate;
+    a.scale = ldexp(1.0, g.bits);
+    a.inv_scale = ldexp(1.0, -g.bits);
+    switch (default_variant(0, g.H)) {
+        case 1: return forward_single<1>(a, s);
+        case 2: return forward_single<2>(a, s);
+        default: return forward_single<4>(a, s);
+    }
+}
+
+hipError_t launch_forward_batched(int variant, const BatchItem* items_host,
+                                  const BatchItem* items_dev, int n, int grid, hipStream_t s)
+{
+    switch (variant) {
+        case 1: return forward_batched<1>(items_host, items_dev, n, grid, s);
+        case 2: return forward_batched<2>(items_host, items_dev, n, grid, s);
+        case 4: return forward_batched<4>(items_host, items_dev, n, grid, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+int64_t forward_blocks(int variant, int A, int64_t H, int64_t PX)
+{
+    int nbp = 0, nba = 0, nbh = 0;
+    switch (variant) {
+        case 1: forward_grid<FwdCfg<1>::P, FwdCfg<1>::NW, 1>(A, H, PX, nbp, nba, nbh); break;
+        case 2: forward_grid<FwdCfg<2>::P, FwdCfg<2>::NW, 2>(A, H, PX, nbp, nba, nbh); break;
+        case 4: forward_grid<FwdCfg<4>::P, FwdCfg<4>::NW, 4>(A, H, PX, nbp, nba, nbh); break;
+        default: return -1;
+    }
+    return (int64_t)nbp * nba * nbh;
 }
 
 hipError_t launch_transpose(const float* in, float* out, int64_t R, int64_t C, int accumulate,

@@ -12,6 +12,40 @@ struct Geom {
     int bits;           // fractional bits of the cos/sin hi split
 };
 
+// Arguments of one projector instance as the per-block device code sees them.
+struct OpArgs {
+    const float* in;
+    float* out;
+    const float* angles;
+    int A, SY, SX, H, PX;
+    int accumulate;  // 1: out += op(in), 0: out = op(in)
+    double scale, inv_scale;  // 2^B and 2^-B of the cos/sin hi split
+};
+
+// One instance of a batched launch (layout shared with the C ABI's vp_batch_item).
+struct BatchItem {
+    const float* in;
+    float* out;
+    const float* angles;
+    int32_t A, SY, SX, H, PX;
+    int32_t accumulate;
+};
+
+// Slices per lane of the kernel variants (the autotuner's search space).
+constexpr int kForwardVariants[] = {1, 2, 4};
+constexpr int kBackwardVariants[] = {1, 2};
+int default_variant(int op, int64_t H);  // op 0 = A, 1 = AT
+// Work items (blocks) of one instance under a variant; -1 for an unknown variant.
+int64_t forward_blocks(int variant, int A, int64_t H, int64_t PX);
+int64_t backward_blocks(int variant, int64_t SY, int64_t SX, int64_t H);
+
+// Persistent launch over n instances (z_order 0); items_dev holds the same
+// descriptors as items_host.
+hipError_t launch_forward_batched(int variant, const BatchItem* items_host,
+                                  const BatchItem* items_dev, int n, int grid, hipStream_t s);
+hipError_t launch_backward_batched(int variant, const BatchItem* items_host,
+                                   const BatchItem* items_dev, int n, int grid, hipStream_t s);
+
 // x (+)= AT y in z_order 0 (accumulate != 0 adds, else overwrites).
 hipError_t launch_backward_z0(const float* y, float* x, const float* angles, const Geom& g,
                               int accumulate, hipStream_t s);
