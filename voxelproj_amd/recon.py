@@ -64,8 +64,8 @@ class SIRT:
         ones_s = torch.ones(self.sshape, device=device, dtype=dtype)
         rs = ops.project(ones_v, self.angles, self.PX, z_order)
         cs = ops.backproject(ones_s, self.angles, SY, SX, z_order)
-        self.R = torch.where(rs > 1e-6, 1.0 / rs.clamp_min(1e-6), torch.zeros_like(rs))
-        self.C = torch.where(cs > 1e-6, 1.0 / cs.clamp_min(1e-6), torch.zeros_like(cs))
+        self.R = rs.clamp_min(1e-6).reciprocal_().mul_(rs > 1e-6)
+        self.C = cs.clamp_min(1e-6).reciprocal_().mul_(cs > 1e-6)
         self._ax = torch.empty(self.sshape, device=device, dtype=dtype)
         self._bx = torch.empty(self.vshape, device=device, dtype=dtype)
 
