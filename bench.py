@@ -15,6 +15,10 @@ Also reported:
   cpu_baseline  the OpenMP FP32 transcription of the reference algorithm
                 (oracle/, "port") timed on this host on a bounded sample
   traffic       HBM bytes per launch when a PMC summary is supplied
+  batched       48 uneven volumes: plain one-at-a-time A+AT vs the autotuned
+                persistent batched A+AT replayed from a HIP graph
+  serving       Poisson stream of A / AT / SIRT-10 requests: continuous
+                batching (p50/p99 latency, throughput) vs one at a time
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -99,6 +103,78 @@ def cpu_baseline(N, A, H_sample, budget_s=10.0):
     }
 
 
+def bench_batched(dev, n=48, seed=0, reps=3):
+    """Batched form (uneven volumes): plain one-instance-at-a-time A then AT vs
+    the autotuned persistent batched A then AT replayed from a HIP graph."""
+    from voxelproj_amd import batch, ops
+
+    inst = batch.uneven_workload(n, seed=seed, device=dev)
+    xs = [x for x, _ in inst]
+    angs = [a for _, a in inst]
+    ys = [torch.empty((a.numel(), max(x.shape[0], x.shape[1]), x.shape[2]), device=dev)
+          for x, a in inst]
+    xb = [torch.empty_like(x) for x in xs]
+    work = 2.0 * sum(x.shape[0] * x.shape[1] * x.shape[2] * a.numel() for x, a in inst)
+    stream = torch.cuda.current_stream(dev)
+
+    def plain():
+        for x, a, y, b in zip(xs, angs, ys, xb):
+            y.zero_()
+            ops.project(x, a, y.shape[1], out=y)
+            b.zero_()
+            ops.backproject(y, a, x.shape[0], x.shape[1], out=b)
+
+    plain()
+    ms_plain = time_op(plain, (), reps, stream)
+    pa = batch.BatchPlan("A", xs, ys, angs)
+    pb = batch.BatchPlan("AT", ys, xb, angs)
+    tune_a = batch.autotune(pa, persist=False)
+    tune_b = batch.autotune(pb, persist=False)
+    graph = batch.capture_pair(pa, pb)
+    graph.replay()
+    ms_batch = time_op(graph.replay, (), reps, stream)
+    return {
+        "workload": f"{n} volumes, SX=SY in {{96..384}}, H in {{32..256}}, "
+                    f"n_angles in {{60..1024}}, A then AT (z_order 0)",
+        "plain_ms": round(ms_plain, 3),
+        "batched_graph_ms": round(ms_batch, 3),
+        "plain_gvox_ang_s": round(work / ms_plain / 1e6, 1),
+        "batched_gvox_ang_s": round(work / ms_batch / 1e6, 1),
+        "speedup": round(ms_plain / ms_batch, 3),
+        "variants": {"A": pa.variant, "AT": pb.variant},
+        "autotune_ms": {"A": {str(k): round(v, 3) for k, v in tune_a.items()},
+                        "AT": {str(k): round(v, 3) for k, v in tune_b.items()}},
+    }
+
+
+def bench_serving(dev, n=40, rate=200.0, seed=0):
+    """Serving form: Poisson stream (rate req/s) of 40% A / 40% AT / 20% SIRT-10
+    requests; continuous batching vs one request at a time."""
+    from voxelproj_amd import serve
+
+    reqs = serve.mixed_workload(n, rate=rate, seed=seed, device=dev)
+    warm = serve.mixed_workload(4, rate=rate, seed=seed + 1, sizes=(64,), heights=(64,),
+                                n_angles=(90,), device=dev)
+    serve.Server(device=dev).run(warm)
+    server = serve.Server(device=dev)
+    server.run(reqs)
+    st = serve.latency_stats(reqs)
+    _, t_plain = serve.run_plain(reqs)
+    served = serve.throughput(reqs, server.makespan)
+    plain = serve.throughput(reqs, t_plain)
+    return {
+        "workload": f"{n} requests, Poisson {rate:g} req/s, 40% A / 40% AT / 20% SIRT-10, "
+                    f"SX=SY in {{128,256,512}}, H in {{64..512}}, n_angles in {{90..1440}}",
+        "p50_ms": round(st["p50_ms"], 2),
+        "p99_ms": round(st["p99_ms"], 2),
+        "makespan_s": round(server.makespan, 3),
+        "ticks": server.ticks,
+        "served_gvox_ang_s": round(served, 1),
+        "plain_gvox_ang_s": round(plain, 1),
+        "speedup": round(served / plain, 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,6 +188,8 @@ def main():
                     help="short run for rocprofv3 counters: one 512 MiB copy (reference "
                          "traffic), then one A and one AT launch; prints no JSON")
     ap.add_argument("--no-z2", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the projector path is not sharded across GPUs "
@@ -229,6 +307,10 @@ def main():
         "roofline": roofline,
         "per_op": per_op,
     }
+    if not args.no_batched:
+        out["batched"] = bench_batched(dev)
+    if not args.no_serving:
+        out["serving"] = bench_serving(dev)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(N, A, args.cpu_slices)
     print(json.dumps(out), flush=True)
