@@ -283,10 +283,16 @@ def main():
         "hbm_algorithmic_GBs": round(dom_bytes / (dom_ms * 1e-3) / 1e9, 1),
         "hbm_frac": round(dom_bytes / (dom_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
     }
-    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(traffic_file):
+    # HBM bytes per launch measured with rocprofv3 PMC runs (FETCH_SIZE, WRITE_SIZE,
+    # scaled by a copy of known size); recorded for this workload only.
+    traffic_file = os.path.join(ROOT, "profiles", "traffic_512cube_a192.txt")
+    if N == 512 and A == 192 and os.path.exists(traffic_file):
+        tr = {}
         with open(traffic_file) as f:
-            tr = json.load(f)
+            for line in f:
+                parts = line.split()
+                if len(parts) == 2 and not line.startswith("#"):
+                    tr[parts[0]] = float(parts[1])
         roofline["traffic"] = tr.get("forward_z0" if ms_a >= ms_at else "backproject_z0")
 
     out = {
