@@ -26,7 +26,6 @@
 namespace vp {
 namespace {
 
-constexpr int kXcds = 8;  // MI355X dispatches workgroup i to XCD i % 8
 
 __device__ __forceinline__ float rdlane(float v, int l)
 {
@@ -241,13 +240,11 @@ __device__ __forceinline__ void backproject_block(const OpArgs& g, const int log
     }
 }
 
-// XCD-aware order: XCD k runs a contiguous range of logical tiles, so
-// neighbouring tiles (which share sinogram rows) meet in the same L2.
-__device__ __forceinline__ int xcd_logical(int bid, int nb)
-{
-    const int per = nb / kXcds;
-    return (bid < per * kXcds) ? (bid % kXcds) * per + bid / kXcds : bid;
-}
+// Block order: consecutive workgroups (round-robin over the 8 XCDs) are
+// neighbouring tiles of one slice chunk, so concurrently running tiles read
+// overlapping sinogram rows, served from the shared Infinity Cache.
+// (Giving each XCD a contiguous tile range instead was measured slower.)
+__device__ __forceinline__ int xcd_logical(int bid, int /*nb*/) { return bid; }
 
 __device__ __forceinline__ OpArgs item_args(const BatchItem& it)
 {
