@@ -222,18 +222,54 @@ __device__ __forceinline__ void backproject_block(const OpArgs& g, const int log
         __syncthreads();
     }
 
+    // Write-out.  The old values of a batch of voxels are loaded first, all
+    // unconditionally (clamped to valid addresses), so their latencies overlap
+    // and no store waits on a load of another branch; the stores of a wave
+    // whose slices all exist are unmasked.
+    constexpr int WB = 8;  // voxels per batch
+    static_assert(VPW % WB == 0, "batch");
+    const bool hfull = hb + VOX <= H;
 #pragma unroll
-    for (int v = 0; v < VPW; ++v) {
-        const int wv = wave * VPW + v;
-        const int ix = tx0 + wv % TX;
-        const int iy = ty0 + wv / TX;
-        if (ix < SX && iy < SY) {
+    for (int v0 = 0; v0 < VPW; v0 += WB) {
+        float old[WB][HL];
+        int64_t base[WB];
+        bool vin[WB];
 #pragma unroll
-            for (int j = 0; j < HL; ++j) {
-                const int hh = hb + lane * HL + j;
-                if (hh < H) {
-                    float* px = x + ((int64_t)iy * SX + ix) * H + hh;
-                    *px = accumulate ? (*px + acc[v][j]) : acc[v][j];
+        for (int v = 0; v < WB; ++v) {
+            const int wv = wave * VPW + v0 + v;
+            const int ix = tx0 + wv % TX;
+            const int iy = ty0 + wv / TX;
+            vin[v] = ix < SX && iy < SY;
+            base[v] = vin[v] ? ((int64_t)iy * SX + ix) * H : 0;
+        }
+        if (accumulate) {
+#pragma unroll
+            for (int v = 0; v < WB; ++v)
+#pragma unroll
+                for (int j = 0; j < HL; ++j) old[v][j] = x[base[v] + min(hb + lane * HL + j, H - 1)];
+        } else {
+#pragma unroll
+            for (int v = 0; v < WB; ++v)
+#pragma unroll
+                for (int j = 0; j < HL; ++j) old[v][j] = 0.0f;
+        }
+        if (hfull) {
+#pragma unroll
+            for (int v = 0; v < WB; ++v) {
+                if (vin[v]) {
+#pragma unroll
+                    for (int j = 0; j < HL; ++j) x[base[v] + hb + lane * HL + j] = old[v][j] + acc[v0 + v][j];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < WB; ++v) {
+                if (vin[v]) {
+#pragma unroll
+                    for (int j = 0; j < HL; ++j) {
+                        const int hh = hb + lane * HL + j;
+                        if (hh < H) x[base[v] + hh] = old[v][j] + acc[v0 + v][j];
+                    }
                 }
             }
         }

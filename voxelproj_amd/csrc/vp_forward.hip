@@ -47,10 +47,10 @@ struct Task {
 // the NW line segments that touch the block's pixels is staged in LDS once
 // (HL slices per lane, 256*HL bytes per voxel) and read by all NW waves, so
 // each volume load feeds up to NW angles.  Lines are split into pieces of at
-// most UMAX voxels; the next piece is fetched into registers while the
-// current one is processed (double-buffered LDS, one barrier per piece).
-// Angles whose dominant axis is x and those whose dominant axis is y are
-// handled in two passes over the volume.
+// most UMAX voxels; while one piece is swept, the next streams into the other
+// LDS buffer by DMA and each wave builds its footprint slot table for it (one
+// barrier per piece).  Angles whose dominant axis is x and those whose
+// dominant axis is y are handled in two passes over the volume.
 template <int P, int NW, int HL>
 __host__ __device__ inline void forward_grid(int A, int64_t H, int64_t PX, int& nbp, int& nba, int& nbh)
 {
@@ -73,11 +73,17 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     static_assert((P + 5) * 1.4143 + 4 <= kSeg, "segment may exceed one wave");
     constexpr int VOX = 64 * HL;                 // floats per staged voxel
     constexpr int STAGE = UMAX * VOX;            // floats per buffer
-    __shared__ float4 s_x4[2][STAGE / 4];
+    // Staging double buffer as two separate arrays: every access names its
+    // buffer statically, so the compiler can tell that the sweep's LDS reads
+    // of one buffer do not alias the DMA into the other and does not wait for
+    // the DMA before them.
+    __shared__ float4 s_xa[STAGE / 4];
+    __shared__ float4 s_xb[STAGE / 4];
     __shared__ AngleK s_k[NW];
     __shared__ int s_mode[NW];
-    // Per wave: footprint slots [rank][centre pixel q] = (wm, w0, wp, LDS offset).
-    __shared__ float4 s_t[NW][2][P + 2];
+    // Per wave, double-buffered: footprint slots [rank][centre pixel q] =
+    // (wm, w0, wp, LDS offset).
+    __shared__ float4 s_t[NW][2][2][P + 2];  // [wave][table buffer][rank][q]
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -109,7 +115,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     k.chi = uni(k.chi); k.clo = uni(k.clo); k.shi = uni(k.shi); k.slo = uni(k.slo);
     k.a = uni(k.a); k.b = uni(k.b); k.h = uni(k.h); k.cd = uni(k.cd);
     const int mode = uni(s_mode[wave]);
-    // Lane w < NW describes angle w of the block (for the staging union).
+    // Lane w < NW describes angle w of the block.
     const AngleK kl = s_k[lane < NW ? lane : 0];
     const int model = (lane < NW) ? s_mode[lane] : -1;
 
@@ -171,53 +177,80 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
         const float cr = px_ ? yc : xc;
         const int64_t st_along = px_ ? (int64_t)H : (int64_t)SX * H;
         const int64_t st_row = px_ ? (int64_t)SX * H : (int64_t)H;
+        const float kap = px_ ? (k.chi + k.clo) : (k.shi + k.slo);
+        const float oth = px_ ? (k.shi + k.slo) : (k.chi + k.clo);
+        const float ikap = 1.0f / kap;
+        // Direction in which the centre pixel grows along a line.
+        const bool fwd = kap >= 0.0f;
+        // Lane w < NW: the (kap, 1/kap, oth) of angle w of the block, read
+        // with v_readlane where the union of the block's segments is formed.
+        const float kcl = kl.chi + kl.clo, ksl = kl.shi + kl.slo;
+        const float kp_l = px_ ? kcl : ksl;
+        const float ot_l = px_ ? ksl : kcl;
+        const float ik_l = 1.0f / kp_l;
 
-        // Segment [jlo, jhi] of a line for an angle (kap, oth): voxels whose
-        // centre pixel can lie in [pl, ph] (one pixel of margin each side).
-        auto segment = [&](int row, float kap, float oth, int& jlo, int& jhi) {
-            const float R = (float)row - cr;
-            const float ub = fmaf(-ca, kap, fmaf(R, oth, pc));
-            const float ikap = 1.0f / kap;
-            const float ja = ((float)pl - 1.5f - ub) * ikap;
-            const float jb = ((float)ph + 1.5f - ub) * ikap;
+        // Segment [jlo, jhi] of line R (centred row coordinate) for an angle
+        // (kap, oth): voxels whose centre pixel can lie in [pl, ph], with one
+        // pixel of margin each side.
+        auto segment = [&](float R, float kp, float ikp, float ot, int& jlo, int& jhi) {
+            const float ub = fmaf(-ca, kp, fmaf(R, ot, pc));
+            const float ja = ((float)pl - 1.5f - ub) * ikp;
+            const float jb = ((float)ph + 1.5f - ub) * ikp;
             jlo = max(0, (int)floorf(fminf(ja, jb)) - 1);
             jhi = min(nalong - 1, (int)ceilf(fmaxf(ja, jb)) + 1);
         };
-        // Union of the block's segments on a line (identical in every wave).
-        auto line_union = [&](int row, int& ulo, int& uhi) {
+        // Line table for 64 consecutive lines c0 .. c0+63, lane = line: the
+        // union of the block's segments (what is staged) and this wave's own
+        // segment; `nonempty` marks lines with anything to stage.
+        int c0 = -64;
+        int u_lo = 0, u_hi = -1, s_lo = 0, s_hi = -1;
+        uint64_t nonempty = 0;
+        auto load_lines = [&](int base) {
+            c0 = base;
+            const int row = base + lane;
+            const float R = (float)row - cr;
             int lo = 0x7fffffff, hi = -1;
-            if (model == pass) {
-                const float cck = kl.chi + kl.clo, ssk = kl.shi + kl.slo;
-                int l, h2;
-                segment(row, px_ ? cck : ssk, px_ ? ssk : cck, l, h2);
-                if (h2 >= l) { lo = l; hi = h2; }
+            // not unrolled: keeps the per-angle reads inside the loop (unrolled,
+            // they are hoisted to every caller of the line search)
+#pragma unroll 1
+            for (int w = 0; w < NW; ++w) {
+                if (__builtin_amdgcn_readlane(model, w) == pass) {
+                    int l, h;
+                    segment(R, rdlane(kp_l, w), rdlane(ik_l, w), rdlane(ot_l, w), l, h);
+                    if (h >= l) {
+                        lo = min(lo, l);
+                        hi = max(hi, h);
+                    }
+                }
             }
-#pragma unroll
-            for (int o = 1; o < 8; o <<= 1) {
-                lo = min(lo, __shfl_xor(lo, o));
-                hi = max(hi, __shfl_xor(hi, o));
-            }
-            ulo = uni(lo);
-            uhi = uni(hi);
+            const bool valid = row < nrows;
+            u_lo = lo;
+            u_hi = valid ? hi : -1;
+            nonempty = __ballot(valid && hi >= lo);
+            s_lo = 0;
+            s_hi = -1;
+            if (mine) segment(R, kap, ikap, oth, s_lo, s_hi);
         };
-        // First piece at or after line `row`.
+        // First piece at or after line `row` (pieces hold at most UMAX voxels).
         auto first_task = [&](int row) {
             Task t{-1, 0, -1};
-            for (; row < nrows; ++row) {
-                int ulo, uhi;
-                line_union(row, ulo, uhi);
-                if (uhi >= ulo) {
-                    t.row = row;
-                    t.j0 = ulo;
-                    t.j1 = min(uhi, ulo + UMAX - 1);
+            while (row < nrows) {
+                if (row >= c0 + 64) load_lines(row);
+                const uint64_t m = nonempty & (~0ull << (row - c0));
+                if (m) {
+                    row = c0 + (int)__builtin_ctzll(m);
+                    const int ulo = uni(__builtin_amdgcn_readlane(u_lo, row - c0));
+                    const int uhi = uni(__builtin_amdgcn_readlane(u_hi, row - c0));
+                    t = Task{row, ulo, min(uhi, ulo + UMAX - 1)};
                     break;
                 }
+                row = c0 + 64;
             }
             return t;
         };
+        // The line of `t` is always in the current table (lines only advance).
         auto next_task = [&](const Task& t) {
-            int ulo, uhi;
-            line_union(t.row, ulo, uhi);
+            const int uhi = uni(__builtin_amdgcn_readlane(u_hi, t.row - c0));
             if (t.j1 < uhi) return Task{t.row, t.j1 + 1, min(uhi, t.j1 + UMAX)};
             return first_task(t.row + 1);
         };
@@ -225,7 +258,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
         // instruction moves one voxel's 64*HL slices (16 B per lane when H is a
         // multiple of 4, else HL dword instructions).  Idle lanes past H load a
         // clamped, valid address; their results are never stored.
-        auto fetch = [&](const Task& t, int b) {
+        auto fetch = [&](const Task& t, float4* dst) __attribute__((always_inline)) {
             const int nv = t.j1 - t.j0 + 1;
             const float* base = x + (int64_t)t.row * st_row + (int64_t)t.j0 * st_along;
             if (vec4) {
@@ -239,7 +272,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                     const int v = min(ins * RPI + sub, nv - 1);
                     __builtin_amdgcn_global_load_lds(
                         (const void*)(base + (int64_t)v * st_along + hoff),
-                        (__attribute__((address_space(3))) void*)&s_x4[b][ins * RPI * (VOX / 4)], 16, 0, 0);
+                        (__attribute__((address_space(3))) void*)&dst[ins * RPI * (VOX / 4)], 16, 0, 0);
                 }
             } else {
                 for (int v = wave; v < nv; v += NW) {
@@ -251,171 +284,172 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                         const int hh = min(hb + e, H - 1);
                         __builtin_amdgcn_global_load_lds(
                             (const void*)(base + (int64_t)v * st_along + hh),
-                            (__attribute__((address_space(3))) void*)(reinterpret_cast<float*>(&s_x4[b][v * (VOX / 4)]) + 64 * c),
+                            (__attribute__((address_space(3))) void*)(reinterpret_cast<float*>(&dst[v * (VOX / 4)]) + 64 * c),
                             4, 0, 0);
                     }
                 }
             }
         };
 
-        __syncthreads();  // previous pass's readers of s_x4 are done
+        // Footprints of this wave's voxels in piece t, scattered into slot
+        // table tb: slot (rank, q) holds the rank-th voxel whose centre pixel
+        // is pl + q (at most two per pixel); empty slots keep zero weights and
+        // a valid dummy offset, so the sweep is straight-line code.  Returns
+        // whether the wave has anything to add from this piece.
+        auto setup = [&](const Task& t, int tb) -> int {
+            if (!mine) return 0;
+            const int jlo = uni(__builtin_amdgcn_readlane(s_lo, t.row - c0));
+            const int jhi = uni(__builtin_amdgcn_readlane(s_hi, t.row - c0));
+            const int n = jhi - jlo + 1;
+            if (n <= 0) return 0;
+            const float R = (float)t.row - cr;
+            const int mcl = min(lane, n - 1);
+            const int jm = fwd ? jlo + mcl : jhi - mcl;
+            const int jp = fwd ? jm - 1 : jm + 1;
+            const Taps tp = voxel_taps(px_ ? (float)jm - xc : R, px_ ? R : (float)jm - yc, pc, k);
+            const int rprev = voxel_r(px_ ? (float)jp - xc : R, px_ ? R : (float)jp - yc, pc, k);
+            const bool valid = lane < n;
+            const int r = valid ? tp.r : 0x3fffffff;
+            const int mm0 = uni((int)__popcll(__ballot(valid && r < pl)));
+            const int mend = uni((int)__popcll(__ballot(valid && r <= ph)));
+            int mA = fwd ? t.j0 - jlo : jhi - t.j1;
+            int mB = fwd ? t.j1 - jlo : jhi - t.j0;
+            mA = uni(max(mA, mm0));
+            mB = uni(min(mB, mend - 1));
+            if (mA > mB) return 0;
+            float4* tab = &s_t[wave][tb][0][0];
+            for (int i = lane; i < 2 * (P + 2); i += 64) tab[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int jb0 = fwd ? (jlo - t.j0) : (jhi - t.j0);
+            const int dj = fwd ? 1 : -1;
+            if (lane >= mA && lane <= mB) {
+                const int q = r - pl;
+                const int rank = (lane > mA && r == rprev) ? 1 : 0;
+                const int off = (jb0 + dj * lane) * VOX;
+                tab[rank * (P + 2) + q] = make_float4(tp.wm, tp.w0, tp.wp, __builtin_bit_cast(float, off));
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            return 1;
+        };
+
+        // Pipeline over pieces: while one piece is swept from one buffer, the
+        // next streams into the other buffer and its slot tables are built.
+        __syncthreads();  // previous pass's readers of s_xa / s_xb / s_t are done
         Task cur = first_task(0);
-        if (cur.row >= 0) fetch(cur, 0);
+        int flag = 0;
+        if (cur.row >= 0) {
+            fetch(cur, s_xa);
+            flag = setup(cur, 0);
+        }
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        int buf = 0;
-        int taps_row = -1;
-        Taps tp{0, 0.f, 0.f, 0.f};
-        int r = 0, rprev_l = 0, jlo = 0, jhi = -1, mm0 = 0, mend = 0;
-        bool fwd = true;
         int rows_done = 0;
-        while (cur.row >= 0) {
+        // One piece: sweep `cur` from xcur with slot table tb; stage the next
+        // piece into xnxt / table tb^1.  Inlined at two call sites so that
+        // both buffers are compile-time known.
+        auto step = [&](const float4* xcur, float4* xnxt, const int tb) __attribute__((always_inline)) {
+            // The table stores come before the DMA is issued, so nothing
+            // between the DMA and the barrier has to wait for it.
             const Task nxt = next_task(cur);
-            if (nxt.row >= 0) fetch(nxt, buf ^ 1);
-            if (cur.row != taps_row) {
-                // New line: fold partial sums every kRowsPerFold lines, then
-                // this wave's footprints.
-                if (cur.row / kRowsPerFold != rows_done / kRowsPerFold) fold();
-                rows_done = cur.row;
-                taps_row = cur.row;
-                mend = mm0 = 0;
-                if (mine) {
-                    const float kap = px_ ? (k.chi + k.clo) : (k.shi + k.slo);
-                    const float oth = px_ ? (k.shi + k.slo) : (k.chi + k.clo);
-                    fwd = kap >= 0.0f;
-                    segment(cur.row, kap, oth, jlo, jhi);
-                    jlo = uni(jlo);
-                    jhi = uni(jhi);
-                    const int n = jhi - jlo + 1;
-                    if (n > 0) {
-                        const float R = (float)cur.row - cr;
-                        const int mcl = min(lane, n - 1);
-                        const int jm = fwd ? jlo + mcl : jhi - mcl;
-                        const int jp = fwd ? jm - 1 : jm + 1;
-                        tp = voxel_taps(px_ ? (float)jm - xc : R, px_ ? R : (float)jm - yc, pc, k);
-                        rprev_l =
-                            voxel_r(px_ ? (float)jp - xc : R, px_ ? R : (float)jp - yc, pc, k);
-
-                        const bool valid = lane < n;
-                        r = valid ? tp.r : 0x3fffffff;
-                        mm0 = uni((int)__popcll(__ballot(valid && r < pl)));
-                        mend = uni((int)__popcll(__ballot(valid && r <= ph)));
+            int flag_n = 0;
+            if (nxt.row >= 0) {
+                flag_n = setup(nxt, tb ^ 1);
+                fetch(nxt, xnxt);
+            }
+            if (flag) {
+                const float4* tab = &s_t[wave][tb][0][0];
+                const int ql = min(lane, P + 1);
+                const float4 t0 = tab[ql];
+                const float4 t1 = tab[(P + 2) + ql];
+                // bit q: pixel pl+q holds a second voxel (about 1 pixel in 4)
+                const uint64_t has2 = __ballot(lane < P + 2 && (t1.x != 0.f || t1.y != 0.f || t1.z != 0.f));
+                const int dvox = fwd ? VOX : -VOX;
+                const float* sx = reinterpret_cast<const float*>(xcur) + lane * HL;
+                auto ld = [&](int off) {
+                    const float* src = sx + off;
+                    struct V { float a[HL]; } out;
+                    if constexpr (HL == 4) {
+                        const float4 t4 = *reinterpret_cast<const float4*>(src);
+                        out.a[0] = t4.x;
+                        out.a[1] = t4.y;
+                        out.a[2] = t4.z;
+                        out.a[3] = t4.w;
+                    } else if constexpr (HL == 2) {
+                        const float2 t2 = *reinterpret_cast<const float2*>(src);
+                        out.a[0] = t2.x;
+                        out.a[1] = t2.y;
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < HL; ++j) out.a[j] = src[j];
+                    }
+                    return out;
+                };
+                // Software-pipelined sweep: the two slot rows of pixel q+1 are
+                // fetched (unconditionally, clamped to the staged buffer)
+                // before the FMAs of pixel q.
+                constexpr int kMaxOff = (UMAX - 1) * VOX;
+                auto slot_offs = [&](int q, int& o0, int& o1) {
+                    o0 = __builtin_bit_cast(int, rdlane(t0.w, q));
+                    o1 = min(max(o0 + dvox, 0), kMaxOff);
+                };
+                int o0, o1;
+                slot_offs(0, o0, o1);
+                auto xa_n = ld(o0);
+                auto xb_n = ld(o1);
+#pragma unroll
+                for (int q = 0; q < P + 2; ++q) {
+                    const auto xa = xa_n;
+                    const auto xb = xb_n;
+                    if (q + 1 < P + 2) {
+                        slot_offs(q + 1, o0, o1);
+                        xa_n = ld(o0);
+                        xb_n = ld(o1);
+                    }
+                    {
+                        const float wm = rdlane(t0.x, q);
+                        const float w0 = rdlane(t0.y, q);
+                        const float wp = rdlane(t0.z, q);
+#pragma unroll
+                        for (int j = 0; j < HL; ++j) {
+                            if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wm, xa.a[j], part[q - 2][j]);
+                            if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(w0, xa.a[j], part[q - 1][j]);
+                            if (q < P) part[q][j] = fmaf(wp, xa.a[j], part[q][j]);
+                        }
+                    }
+                    if ((has2 >> q) & 1ull) {
+                        const float wm = rdlane(t1.x, q);
+                        const float w0 = rdlane(t1.y, q);
+                        const float wp = rdlane(t1.z, q);
+#pragma unroll
+                        for (int j = 0; j < HL; ++j) {
+                            if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wm, xb.a[j], part[q - 2][j]);
+                            if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(w0, xb.a[j], part[q - 1][j]);
+                            if (q < P) part[q][j] = fmaf(wp, xb.a[j], part[q][j]);
+                        }
                     }
                 }
             }
-            if (mend > mm0) {
-                int mA, mB;
-                if (fwd) {
-                    mA = cur.j0 - jlo;
-                    mB = cur.j1 - jlo;
-                } else {
-                    mA = jhi - cur.j1;
-                    mB = jhi - cur.j0;
-                }
-                mA = uni(max(mA, mm0));
-                mB = uni(min(mB, mend - 1));
-                if (mA <= mB) {
-                    // Scatter this piece's footprints into the slot table:
-                    // slot (rank, q) holds the rank-th voxel whose centre pixel
-                    // is pl + q (at most two per pixel); empty slots keep zero
-                    // weights and a valid dummy offset, so the sweep below is
-                    // straight-line code.
-                    float4* tab = &s_t[wave][0][0];
-                    for (int i = lane; i < 2 * (P + 2); i += 64) tab[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    const int jb0 = fwd ? (jlo - cur.j0) : (jhi - cur.j0);
-                    const int dj = fwd ? 1 : -1;
-                    if (lane >= mA && lane <= mB) {
-                        const int q = r - pl;
-                        const int rank = (lane > mA && r == rprev_l) ? 1 : 0;
-                        const int off = (jb0 + dj * lane) * VOX;
-                        tab[rank * (P + 2) + q] =
-                            make_float4(tp.wm, tp.w0, tp.wp, __builtin_bit_cast(float, off));
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    const int ql = min(lane, P + 1);
-                    const float4 t0 = tab[ql];
-                    const float4 t1 = tab[(P + 2) + ql];
-                    // bit q: pixel pl+q holds a second voxel (about 1 pixel in 4)
-                    const uint64_t has2 = __ballot(lane < P + 2 && (t1.x != 0.f || t1.y != 0.f || t1.z != 0.f));
-                    const int dvox = dj * VOX;
-                    const float* sx = reinterpret_cast<const float*>(s_x4[buf]) + lane * HL;
-                    auto ld = [&](int off) {
-                        const float* src = sx + off;
-                        struct V { float a[HL]; } out;
-                        if constexpr (HL == 4) {
-                            const float4 t4 = *reinterpret_cast<const float4*>(src);
-                            out.a[0] = t4.x;
-                            out.a[1] = t4.y;
-                            out.a[2] = t4.z;
-                            out.a[3] = t4.w;
-                        } else if constexpr (HL == 2) {
-                            const float2 t2 = *reinterpret_cast<const float2*>(src);
-                            out.a[0] = t2.x;
-                            out.a[1] = t2.y;
-                        } else {
-#pragma unroll
-                            for (int j = 0; j < HL; ++j) out.a[j] = src[j];
-                        }
-                        return out;
-                    };
-                    // Software-pipelined sweep: the two slot rows of pixel q+1 are
-                    // fetched (unconditionally, clamped to the staged buffer)
-                    // before the FMAs of pixel q.
-                    constexpr int kMaxOff = (UMAX - 1) * VOX;
-                    auto slot_offs = [&](int q, int& o0, int& o1) {
-                        o0 = __builtin_bit_cast(int, rdlane(t0.w, q));
-                        o1 = min(max(o0 + dvox, 0), kMaxOff);
-                    };
-                    int o0, o1;
-                    slot_offs(0, o0, o1);
-                    auto xa_n = ld(o0);
-                    auto xb_n = ld(o1);
-#pragma unroll
-                    for (int q = 0; q < P + 2; ++q) {
-                        const auto xa = xa_n;
-                        const auto xb = xb_n;
-                        if (q + 1 < P + 2) {
-                            slot_offs(q + 1, o0, o1);
-                            xa_n = ld(o0);
-                            xb_n = ld(o1);
-                        }
-                        {
-                            const float wm = rdlane(t0.x, q);
-                            const float w0 = rdlane(t0.y, q);
-                            const float wp = rdlane(t0.z, q);
-#pragma unroll
-                            for (int j = 0; j < HL; ++j) {
-                                if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wm, xa.a[j], part[q - 2][j]);
-                                if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(w0, xa.a[j], part[q - 1][j]);
-                                if (q < P) part[q][j] = fmaf(wp, xa.a[j], part[q][j]);
-                            }
-                        }
-                        if ((has2 >> q) & 1ull) {
-                            const float wm = rdlane(t1.x, q);
-                            const float w0 = rdlane(t1.y, q);
-                            const float wp = rdlane(t1.z, q);
-#pragma unroll
-                            for (int j = 0; j < HL; ++j) {
-                                if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wm, xb.a[j], part[q - 2][j]);
-                                if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(w0, xb.a[j], part[q - 1][j]);
-                                if (q < P) part[q][j] = fmaf(wp, xb.a[j], part[q][j]);
-                            }
-                        }
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                }
-            }
-            __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA into buf^1 has landed
+            __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA into xnxt has landed
             __syncthreads();
-            buf ^= 1;
             cur = nxt;
+            flag = flag_n;
+        };
+        while (cur.row >= 0) {
+            // Fold the partial sums when a new group of kRowsPerFold lines
+            // starts (checked once per pair of pieces, so the fold code exists
+            // once; its global loads retire before the next DMA is issued, as
+            // vmcnt retires in order).
+            if (cur.row / kRowsPerFold != rows_done / kRowsPerFold) {
+                fold();
+                rows_done = cur.row;
+            }
+            step(s_xa, s_xb, 0);
+            if (cur.row < 0) break;
+            step(s_xb, s_xa, 1);
         }
         fold();
     }
