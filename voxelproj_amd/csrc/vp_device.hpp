@@ -99,6 +99,29 @@ __device__ __forceinline__ Taps voxel_taps(float X, float Y, float pc, const Ang
     return o;
 }
 
+// The same footprint anchored at its first nonzero pixel l: pixel l gets wl,
+// l+1 gets wr and l+2 gets w3.  The centre pixel always has a positive weight
+// (it contains the footprint's centre), so l = r-1 when pixel r-1 is hit and
+// l = r otherwise; w3 is nonzero only when the footprint covers three pixels
+// (about one voxel-angle in eight at 512^3 x 192), so the projectors apply
+// two taps always and the third under a wave-uniform test.  Dropping the zero
+// tap leaves every sum unchanged (x*0 adds +-0).
+struct Taps2 {
+    int l;
+    float wl, wr, w3;
+};
+
+__device__ __forceinline__ Taps2 anchor_left(const Taps& t)
+{
+    const bool left = t.wm != 0.0f;
+    Taps2 o;
+    o.l = left ? t.r - 1 : t.r;
+    o.wl = left ? t.wm : t.w0;
+    o.wr = left ? t.w0 : t.wp;
+    o.w3 = left ? t.wp : 0.0f;
+    return o;
+}
+
 // Centre pixel only, for window and segment bounds.
 __device__ __forceinline__ int voxel_r(float X, float Y, float pc, const AngleK& k)
 {

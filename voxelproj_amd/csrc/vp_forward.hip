@@ -6,13 +6,14 @@
 // into output-stationary accumulators:
 //   wave  = one angle x P consecutive detector pixels x 64 H slices (lane = slice);
 //   rows  = lines of the volume along the axis whose direction cosine is
-//           larger (|kappa| >= 1/sqrt 2), so at most two voxels of a line
-//           share a centre pixel r and consecutive voxels step r by 0, 1
-//           (or, within rounding near 0 deg, 2);
+//           larger (|kappa| >= 1/sqrt 2), so consecutive voxels of a line
+//           are at least 1/sqrt 2 pixel apart on the detector and at most
+//           two of them share the first pixel l of their footprint;
 //   per line, lane m evaluates the footprint of the m-th voxel of the line
-//   segment that touches the wave's pixels (in increasing-r order), and a
-//   statically unrolled sweep over r = p0-1 .. p0+P adds each voxel to
-//   accumulators r-1, r, r+1 -- so every accumulator index is a
+//   segment that touches the wave's pixels (in increasing-u order), and a
+//   statically unrolled sweep over l = p0-2 .. p0+P-1 adds each voxel to
+//   accumulators l, l+1 and, for the one voxel-angle in eight whose
+//   footprint covers three pixels, l+2 -- so every accumulator index is a
 //   compile-time constant and only the voxel index is dynamic.
 // No atomics; the summation order is fixed, so results are deterministic.
 #include "vp_device.hpp"
@@ -81,8 +82,8 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     __shared__ float4 s_xb[STAGE / 4];
     __shared__ AngleK s_k[NW];
     __shared__ int s_mode[NW];
-    // Per wave, double-buffered: footprint slots [rank][centre pixel q] =
-    // (wm, w0, wp, LDS offset).
+    // Per wave, double-buffered: footprint slots [rank][first pixel q] =
+    // (wl, wr, w3, LDS offset).
     __shared__ float4 s_t[NW][2][2][P + 2];  // [wave][table buffer][rank][q]
 
     const int tid = threadIdx.x;
@@ -292,10 +293,10 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
         };
 
         // Footprints of this wave's voxels in piece t, scattered into slot
-        // table tb: slot (rank, q) holds the rank-th voxel whose centre pixel
-        // is pl + q (at most two per pixel); empty slots keep zero weights and
-        // a valid dummy offset, so the sweep is straight-line code.  Returns
-        // whether the wave has anything to add from this piece.
+        // table tb: slot (rank, q) holds the rank-th voxel whose footprint
+        // starts at pixel p0 - 2 + q (at most two per pixel); empty slots keep
+        // zero weights and a valid dummy offset, so the sweep is straight-line
+        // code.  Returns whether the wave has anything to add from this piece.
         auto setup = [&](const Task& t, int tb) -> int {
             if (!mine) return 0;
             const int jlo = uni(__builtin_amdgcn_readlane(s_lo, t.row - c0));
@@ -305,13 +306,15 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
             const float R = (float)t.row - cr;
             const int mcl = min(lane, n - 1);
             const int jm = fwd ? jlo + mcl : jhi - mcl;
-            const int jp = fwd ? jm - 1 : jm + 1;
-            const Taps tp = voxel_taps(px_ ? (float)jm - xc : R, px_ ? R : (float)jm - yc, pc, k);
-            const int rprev = voxel_r(px_ ? (float)jp - xc : R, px_ ? R : (float)jp - yc, pc, k);
+            // footprint anchored at its first nonzero pixel l; slot q = l - (pl - 1)
+            // covers l in [p0-2, p0+P-1], the voxels that touch the block's pixels
+            const Taps2 tp = anchor_left(
+                voxel_taps(px_ ? (float)jm - xc : R, px_ ? R : (float)jm - yc, pc, k));
+            const int lprev = __shfl_up(tp.l, 1);  // lane m-1 holds the previous voxel
             const bool valid = lane < n;
-            const int r = valid ? tp.r : 0x3fffffff;
-            const int mm0 = uni((int)__popcll(__ballot(valid && r < pl)));
-            const int mend = uni((int)__popcll(__ballot(valid && r <= ph)));
+            const int l = valid ? tp.l : 0x3fffffff;
+            const int mm0 = uni((int)__popcll(__ballot(valid && l < pl - 1)));
+            const int mend = uni((int)__popcll(__ballot(valid && l <= ph - 1)));
             int mA = fwd ? t.j0 - jlo : jhi - t.j1;
             int mB = fwd ? t.j1 - jlo : jhi - t.j0;
             mA = uni(max(mA, mm0));
@@ -325,10 +328,10 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
             const int jb0 = fwd ? (jlo - t.j0) : (jhi - t.j0);
             const int dj = fwd ? 1 : -1;
             if (lane >= mA && lane <= mB) {
-                const int q = r - pl;
-                const int rank = (lane > mA && r == rprev) ? 1 : 0;
+                const int q = l - (pl - 1);
+                const int rank = (lane > mA && l == lprev) ? 1 : 0;
                 const int off = (jb0 + dj * lane) * VOX;
-                tab[rank * (P + 2) + q] = make_float4(tp.wm, tp.w0, tp.wp, __builtin_bit_cast(float, off));
+                tab[rank * (P + 2) + q] = make_float4(tp.wl, tp.wr, tp.w3, __builtin_bit_cast(float, off));
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -365,8 +368,11 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                 const int ql = min(lane, P + 1);
                 const float4 t0 = tab[ql];
                 const float4 t1 = tab[(P + 2) + ql];
-                // bit q: pixel pl+q holds a second voxel (about 1 pixel in 4)
+                // bit q: slot q holds a second voxel (about 1 in 10 at 512^3 x 192)
                 const uint64_t has2 = __ballot(lane < P + 2 && (t1.x != 0.f || t1.y != 0.f || t1.z != 0.f));
+                // bit q: the rank-0 / rank-1 voxel of slot q covers three pixels
+                const uint64_t has3a = __ballot(lane < P + 2 && t0.z != 0.f);
+                const uint64_t has3b = __ballot(lane < P + 2 && t1.z != 0.f);
                 const int dvox = fwd ? VOX : -VOX;
                 const float* sx = reinterpret_cast<const float*>(xcur) + lane * HL;
                 auto ld = [&](int off) {
@@ -410,25 +416,31 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                         xb_n = ld(o1);
                     }
                     {
-                        const float wm = rdlane(t0.x, q);
-                        const float w0 = rdlane(t0.y, q);
-                        const float wp = rdlane(t0.z, q);
+                        const float wl = rdlane(t0.x, q);
+                        const float wr = rdlane(t0.y, q);
 #pragma unroll
                         for (int j = 0; j < HL; ++j) {
-                            if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wm, xa.a[j], part[q - 2][j]);
-                            if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(w0, xa.a[j], part[q - 1][j]);
-                            if (q < P) part[q][j] = fmaf(wp, xa.a[j], part[q][j]);
+                            if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wl, xa.a[j], part[q - 2][j]);
+                            if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(wr, xa.a[j], part[q - 1][j]);
+                        }
+                        if (q < P && ((has3a >> q) & 1ull)) {
+                            const float w3 = rdlane(t0.z, q);
+#pragma unroll
+                            for (int j = 0; j < HL; ++j) part[q][j] = fmaf(w3, xa.a[j], part[q][j]);
                         }
                     }
                     if ((has2 >> q) & 1ull) {
-                        const float wm = rdlane(t1.x, q);
-                        const float w0 = rdlane(t1.y, q);
-                        const float wp = rdlane(t1.z, q);
+                        const float wl = rdlane(t1.x, q);
+                        const float wr = rdlane(t1.y, q);
 #pragma unroll
                         for (int j = 0; j < HL; ++j) {
-                            if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wm, xb.a[j], part[q - 2][j]);
-                            if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(w0, xb.a[j], part[q - 1][j]);
-                            if (q < P) part[q][j] = fmaf(wp, xb.a[j], part[q][j]);
+                            if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wl, xb.a[j], part[q - 2][j]);
+                            if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(wr, xb.a[j], part[q - 1][j]);
+                        }
+                        if (q < P && ((has3b >> q) & 1ull)) {
+                            const float w3 = rdlane(t1.z, q);
+#pragma unroll
+                            for (int j = 0; j < HL; ++j) part[q][j] = fmaf(w3, xb.a[j], part[q][j]);
                         }
                     }
                 }
