@@ -66,6 +66,46 @@ VP_API int vp_backward_f64(int z_order, const double *y, double *x, const float 
                            int64_t SY, int64_t SX, int64_t H, int64_t PX, void *stream);
 
 /*
+ * Translated detector geometry (the reference lists detector translations as
+ * future work: README.md:65-68; no counterpart in its code).  Same as
+ * vp_forward / vp_backward and the _f64 variants, plus two optional device
+ * arrays of n_angles float32 values (NULL = no translation):
+ *   du: detector translation along the pixel axis, in pixels; pixel p of angle
+ *       a then sees u = (ix-xc) cos + (iy-yc) sin + (PX-1)/2 - du[a];
+ *   dv: detector translation along H, in slices; sinogram slice h of angle a
+ *       then samples the volume at slice position h + dv[a] (linear
+ *       interpolation between slices, zero outside [0, H)).
+ * du_max must bound max |du[a]| (it sizes the exact position split; ignored
+ * when du is NULL).  The backprojector is the exact adjoint.  A translation
+ * (dx, dy) of the volume in its plane equals du[a] = -(dx cos a + dy sin a).
+ */
+VP_API int vp_forward_geom(int z_order, const float *x, float *y, const float *angles, int n_angles,
+                           int64_t SY, int64_t SX, int64_t H, int64_t PX, const float *du,
+                           const float *dv, double du_max, void *stream);
+VP_API int vp_backward_geom(int z_order, const float *y, float *x, const float *angles, int n_angles,
+                            int64_t SY, int64_t SX, int64_t H, int64_t PX, const float *du,
+                            const float *dv, double du_max, void *stream);
+VP_API int vp_forward_geom_f64(int z_order, const double *x, double *y, const float *angles,
+                               int n_angles, int64_t SY, int64_t SX, int64_t H, int64_t PX,
+                               const float *du, const float *dv, double du_max, void *stream);
+VP_API int vp_backward_geom_f64(int z_order, const double *y, double *x, const float *angles,
+                                int n_angles, int64_t SY, int64_t SX, int64_t H, int64_t PX,
+                                const float *du, const float *dv, double du_max, void *stream);
+
+/*
+ * The H translation on its own, on slabs of a sinogram, for drivers that
+ * stream data larger than device memory through the projectors in H slabs
+ * (no counterpart in the reference).  `in` holds global slices
+ * [in_offset, in_offset + H_in), `out` [out_offset, out_offset + H_out);
+ * transpose 0 gives out(h) = interpolation of in at h + dv[a] (the forward
+ * translation), transpose 1 its adjoint.  Input slices outside the slab read
+ * zero; accumulate 1 adds to out, 0 overwrites it.
+ */
+VP_API int vp_vshift(int z_order, int transpose, const float *in, float *out, int n_angles,
+                     int64_t PX, int64_t H_in, int64_t in_offset, int64_t H_out,
+                     int64_t out_offset, const float *dv, int accumulate, void *stream);
+
+/*
  * Batched form: many projector instances of uneven size and angle count in
  * one persistent launch (z_order 0 layout).  No counterpart in the
  * reference, which projects one volume per call.

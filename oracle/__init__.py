@@ -15,6 +15,11 @@ ref_forward_f32 / ref_backward_f32
     FP32 transcription of the reference CUDA kernels' arithmetic
     (voxelproj/csrc/include/sp_forward_z0.cuh:8-185, sp_backward_z0.cuh:8-100
     and their z_order 2 twins), standing in for the CUDA operator.
+forward_geom_f64 / backward_geom_f64
+    The exact operator with detector translations du (pixels, along the
+    detector) and dv (slices, along H, linear interpolation), written from
+    the definition (an extension: the reference lists translations as future
+    work, README.md:65-68).
 nnz
     Exact nonzero count of W, for the algorithmic flop count 2*H*nnz.
 
@@ -26,6 +31,8 @@ from .oracle import (  # noqa: F401
     build,
     forward_f64,
     backward_f64,
+    forward_geom_f64,
+    backward_geom_f64,
     ref_forward_f32,
     ref_backward_f32,
     nnz,

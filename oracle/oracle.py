@@ -34,6 +34,11 @@ def load():
     lib.vpo_forward_f64.restype = _int
     lib.vpo_backward_f64.argtypes = sig
     lib.vpo_backward_f64.restype = _int
+    sigg = sig[:-1] + [_vp, _vp, _int]
+    lib.vpo_forward_geom_f64.argtypes = sigg
+    lib.vpo_forward_geom_f64.restype = _int
+    lib.vpo_backward_geom_f64.argtypes = sigg
+    lib.vpo_backward_geom_f64.restype = _int
     sig32 = [_vp, _vp, _vp, _int, _i64, _i64, _i64, _i64, _int, _i64, _i64, _int]
     lib.vpo_ref_forward_f32.argtypes = sig32
     lib.vpo_ref_forward_f32.restype = _int
@@ -135,6 +140,61 @@ def backward_f64(y, angles, SY=None, SX=None, z_order=0, h0=0, h1=None, nthreads
                               SY, SX, H, PX, z_order, h0, h1, nthreads)
     if rc:
         raise RuntimeError(f"vpo_backward_f64 failed ({rc})")
+    return x
+
+
+def _shift64(v, n, name):
+    """None, a number or n values -> None or float64 array of n values.  The
+    values are rounded to float32 first, as the device library receives them."""
+    if v is None:
+        return None
+    a = np.asarray(v, dtype=np.float32).astype(np.float64).reshape(-1)
+    if a.size == 1:
+        a = np.full(n, a[0])
+    if a.size != n:
+        raise ValueError(f"{name} needs 1 or {n} values")
+    return np.ascontiguousarray(a)
+
+
+def forward_geom_f64(x, angles, PX=None, z_order=0, du=None, dv=None, h0=0, h1=None, nthreads=0):
+    """Exact A x in FP64 with detector translations du (pixels) / dv (slices)."""
+    lib = load()
+    xin, is64 = _input(x)
+    SY, SX, H = _vol_dims(xin, z_order)
+    a = _angles32(angles)
+    PX = max(SY, SX) if PX is None else PX
+    h0, h1 = _hrange(H, h0, h1)
+    u, v = _shift64(du, a.size, "du"), _shift64(dv, a.size, "dv")
+    y = np.zeros(y_shape(z_order, a.size, PX, H), np.float64)
+    rc = lib.vpo_forward_geom_f64(xin.ctypes.data, is64, y.ctypes.data, a.ctypes.data, a.size,
+                                  SY, SX, H, PX, z_order, h0, h1,
+                                  None if u is None else u.ctypes.data,
+                                  None if v is None else v.ctypes.data, nthreads)
+    if rc:
+        raise RuntimeError(f"vpo_forward_geom_f64 failed ({rc})")
+    return y
+
+
+def backward_geom_f64(y, angles, SY=None, SX=None, z_order=0, du=None, dv=None, h0=0, h1=None,
+                      nthreads=0):
+    """Exact AT y in FP64 with detector translations du / dv."""
+    lib = load()
+    yin, is64 = _input(y)
+    A, PX, H = _sino_dims(yin, z_order)
+    a = _angles32(angles)
+    if a.size != A:
+        raise ValueError("len(angles) does not match the sinogram")
+    SY = PX if SY is None else SY
+    SX = PX if SX is None else SX
+    h0, h1 = _hrange(H, h0, h1)
+    u, v = _shift64(du, A, "du"), _shift64(dv, A, "dv")
+    x = np.zeros(x_shape(z_order, SY, SX, H), np.float64)
+    rc = lib.vpo_backward_geom_f64(yin.ctypes.data, is64, x.ctypes.data, a.ctypes.data, a.size,
+                                   SY, SX, H, PX, z_order, h0, h1,
+                                   None if u is None else u.ctypes.data,
+                                   None if v is None else v.ctypes.data, nthreads)
+    if rc:
+        raise RuntimeError(f"vpo_backward_geom_f64 failed ({rc})")
     return x
 
 

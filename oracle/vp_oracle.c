@@ -224,6 +224,114 @@ VPO_EXPORT int vpo_backward_f64(const void *y, int y_is_f64, double *x,
 }
 
 /*
+ * Translated detector (an extension; the reference lists translations as
+ * future work, README.md:65-68), from the definition:
+ *   u_a(ix, iy) = (ix - xc) cos + (iy - yc) sin + pc - du[a],
+ *   sinogram slice h of angle a sees the volume at slice position h + dv[a],
+ *   linearly interpolated: (1 - f) x[., k + h] + f x[., k + h + 1],
+ *   k = floor(dv[a]), f = dv[a] - k, slices outside [0, H) are zero.
+ * du or dv may be NULL (no translation).  Output slices [h0, h1).  The
+ * backprojector is written as the transpose of the same sum (gather over the
+ * sinogram slices that read volume slice j), not as a second derivation.
+ */
+static inline double vpo_vol_at(const void *x, int is_f64, int64_t base, int64_t stride_h,
+                                int64_t H, int64_t j)
+{
+    return (j < 0 || j >= H) ? 0.0 : vpo_load(x, is_f64, base + j * stride_h);
+}
+
+VPO_EXPORT int vpo_forward_geom_f64(const void *x, int x_is_f64, double *y,
+                                    const float *angles, int n_angles,
+                                    int64_t SY, int64_t SX, int64_t H, int64_t PX,
+                                    int z_order, int64_t h0, int64_t h1,
+                                    const double *du, const double *dv, int nthreads)
+{
+    vpo_strides st;
+    if (vpo_make_strides(z_order, SY, SX, H, n_angles, PX, &st)) return -1;
+    if (h0 < 0 || h1 > H || h0 > h1) return -2;
+    const double xc = 0.5 * (double)(SX - 1);
+    const double yc = 0.5 * (double)(SY - 1);
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int ia = 0; ia < n_angles; ++ia) {
+        const vpo_angle64 k = vpo_angle_consts64(angles[ia]);
+        const double pc = 0.5 * (double)(PX - 1) - (du ? du[ia] : 0.0);
+        const double d = dv ? dv[ia] : 0.0;
+        const double kf = floor(d), f = d - kf;
+        const int64_t kv = (int64_t)kf;
+        for (int64_t iy = 0; iy < SY; ++iy) {
+            for (int64_t ix = 0; ix < SX; ++ix) {
+                const double u = ((double)ix - xc) * k.c + ((double)iy - yc) * k.s + pc;
+                const double r = nearbyint(u);
+                for (int t = -1; t <= 1; ++t) {
+                    const double p = r + t;
+                    if (p < 0.0 || p > (double)(PX - 1)) continue;
+                    const double w = vpo_weight64(p, u, &k);
+                    if (w == 0.0) continue;
+                    const int64_t vb = iy * st.vy + ix * st.vx;
+                    const int64_t pb = (int64_t)ia * st.pa + (int64_t)p * st.pp;
+                    for (int64_t hh = h0; hh < h1; ++hh) {
+                        const double xv = (1.0 - f) * vpo_vol_at(x, x_is_f64, vb, st.vh, H, hh + kv) +
+                                          f * vpo_vol_at(x, x_is_f64, vb, st.vh, H, hh + kv + 1);
+                        y[pb + hh * st.ph] += w * xv;
+                    }
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+VPO_EXPORT int vpo_backward_geom_f64(const void *y, int y_is_f64, double *x,
+                                     const float *angles, int n_angles,
+                                     int64_t SY, int64_t SX, int64_t H, int64_t PX,
+                                     int z_order, int64_t h0, int64_t h1,
+                                     const double *du, const double *dv, int nthreads)
+{
+    vpo_strides st;
+    if (vpo_make_strides(z_order, SY, SX, H, n_angles, PX, &st)) return -1;
+    if (h0 < 0 || h1 > H || h0 > h1) return -2;
+    const double xc = 0.5 * (double)(SX - 1);
+    const double yc = 0.5 * (double)(SY - 1);
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+
+    vpo_angle64 *tab = (vpo_angle64 *)__builtin_alloca(sizeof(vpo_angle64) * (size_t)(n_angles > 0 ? n_angles : 1));
+    for (int ia = 0; ia < n_angles; ++ia) tab[ia] = vpo_angle_consts64(angles[ia]);
+
+#pragma omp parallel for collapse(2) schedule(static) num_threads(nthreads)
+    for (int64_t iy = 0; iy < SY; ++iy) {
+        for (int64_t ix = 0; ix < SX; ++ix) {
+            const int64_t vb = iy * st.vy + ix * st.vx;
+            for (int ia = 0; ia < n_angles; ++ia) {
+                const vpo_angle64 *k = &tab[ia];
+                const double pc = 0.5 * (double)(PX - 1) - (du ? du[ia] : 0.0);
+                const double d = dv ? dv[ia] : 0.0;
+                const double kf = floor(d), f = d - kf;
+                const int64_t kv = (int64_t)kf;
+                const double u = ((double)ix - xc) * k->c + ((double)iy - yc) * k->s + pc;
+                const double r = nearbyint(u);
+                for (int t = -1; t <= 1; ++t) {
+                    const double p = r + t;
+                    if (p < 0.0 || p > (double)(PX - 1)) continue;
+                    const double w = vpo_weight64(p, u, k);
+                    if (w == 0.0) continue;
+                    const int64_t pb = (int64_t)ia * st.pa + (int64_t)p * st.pp;
+                    /* volume slice j is read by sinogram slice hh = j - kv
+                     * (weight 1 - f) and hh = j - kv - 1 (weight f) */
+                    for (int64_t j = h0; j < h1; ++j) {
+                        const double g = (1.0 - f) * vpo_vol_at(y, y_is_f64, pb, st.ph, H, j - kv) +
+                                         f * vpo_vol_at(y, y_is_f64, pb, st.ph, H, j - kv - 1);
+                        x[vb + j * st.vh] += w * g;
+                    }
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+/*
  * Exact nonzero count of W over all angles (pixels restricted to [0, PX)),
  * used for the algorithmic flop count 2 * H * nnz.
  */

@@ -38,13 +38,15 @@ def _area(poly):
     return abs(s) / 2.0
 
 
-def weights(phi, SY, SX, PX):
-    """Dense W of shape (PX, SY, SX) for one float32 angle phi."""
+def weights(phi, SY, SX, PX, du=0.0):
+    """Dense W of shape (PX, SY, SX) for one float32 angle phi; du translates
+    the detector along its axis by du pixels (float32-rounded, as the library
+    receives it)."""
     phi = float(np.float32(phi))
     c, s = np.cos(phi), np.sin(phi)
     xc, yc, pc = (SX - 1) / 2.0, (SY - 1) / 2.0, (PX - 1) / 2.0
-    # detector coordinate of a point: (qx - xc) c + (qy - yc) s + pc
-    shift = -xc * c - yc * s + pc
+    # detector coordinate of a point: (qx - xc) c + (qy - yc) s + pc - du
+    shift = -xc * c - yc * s + pc - float(np.float32(du))
     W = np.zeros((PX, SY, SX))
     for iy in range(SY):
         for ix in range(SX):

@@ -73,3 +73,34 @@ def test_empty_problem_is_a_no_op(fn):
     assert _call(fn, ptr=0, A=0) == 0
     assert _call(fn, ptr=0, H=0) == 0
     assert voxelproj_amd.load().vp_last_error() == b""
+
+
+def _call_geom(fn, z=0, ptr=1, A=3, SY=4, SX=4, H=2, PX=5, du=None, dv=None, du_max=0.0):
+    lib = voxelproj_amd.load()
+    p = ctypes.c_void_p(ptr) if ptr else None
+    return getattr(lib, fn)(z, p, p, p, A, SY, SX, H, PX, du, dv, du_max, None)
+
+
+@pytest.mark.parametrize("fn", ["vp_forward_geom", "vp_backward_geom", "vp_forward_geom_f64",
+                                "vp_backward_geom_f64"])
+def test_geometry_entry_points_validate(fn):
+    lib = voxelproj_amd.load()
+    assert _call_geom(fn, z=1) == -1 and b"z_order" in lib.vp_last_error()
+    assert _call_geom(fn, ptr=0) == -1 and b"null" in lib.vp_last_error()
+    assert _call_geom(fn, PX=40000) == -2
+    # du_max must be a finite, non-negative bound
+    assert _call_geom(fn, du=ctypes.c_void_p(1), du_max=-1.0) == -1
+    assert _call_geom(fn, du=ctypes.c_void_p(1), du_max=float("nan")) == -1
+    assert b"du_max" in lib.vp_last_error()
+    assert _call_geom(fn, ptr=0, A=0) == 0
+
+
+def test_vshift_validates():
+    lib = voxelproj_amd.load()
+    p = ctypes.c_void_p(1)
+    assert lib.vp_vshift(1, 0, p, p, 3, 5, 4, 0, 4, 0, p, 0, None) == -1
+    assert lib.vp_vshift(0, 2, p, p, 3, 5, 4, 0, 4, 0, p, 0, None) == -1
+    assert lib.vp_vshift(0, 0, p, p, 3, 5, 4, 0, 4, 0, p, 2, None) == -1
+    assert lib.vp_vshift(0, 0, p, None, 3, 5, 4, 0, 4, 0, p, 0, None) == -1
+    assert lib.vp_vshift(0, 0, p, p, 3, -5, 4, 0, 4, 0, p, 0, None) == -1
+    assert lib.vp_vshift(0, 0, None, None, 3, 5, 4, 0, 0, 0, None, 0, None) == 0  # empty output

@@ -13,9 +13,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libvoxelproj_hip.so")
 
 EXPORTS = ("vp_forward", "vp_backward", "vp_forward_f64", "vp_backward_f64",
-           "vp_forward_batched", "vp_backward_batched", "vp_default_variant", "vp_batch_blocks",
-           "vp_last_error", "vp_abi_version")
-ABI_VERSION = 1
+           "vp_forward_geom", "vp_backward_geom", "vp_forward_geom_f64", "vp_backward_geom_f64",
+           "vp_vshift", "vp_forward_batched", "vp_backward_batched", "vp_default_variant",
+           "vp_batch_blocks", "vp_last_error", "vp_abi_version")
+ABI_VERSION = 2
 
 _lib = None
 _lock = threading.Lock()
@@ -57,6 +58,16 @@ def load():
             f = getattr(lib, name)
             f.argtypes = sig
             f.restype = ctypes.c_int
+        for name in ("vp_forward_geom", "vp_backward_geom", "vp_forward_geom_f64",
+                     "vp_backward_geom_f64"):
+            f = getattr(lib, name)
+            f.argtypes = sig[:-1] + [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                     ctypes.c_void_p]
+            f.restype = ctypes.c_int
+        lib.vp_vshift.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_int, i64, i64, i64, i64, i64, ctypes.c_void_p,
+                                  ctypes.c_int, ctypes.c_void_p]
+        lib.vp_vshift.restype = ctypes.c_int
         for name in ("vp_forward_batched", "vp_backward_batched"):
             f = getattr(lib, name)
             f.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,

@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 OUT = os.path.join(PKG, "libvoxelproj_hip.so")
-SOURCES = ["vp_capi.hip", "vp_backward.hip", "vp_forward.hip", "vp_f64.hip"]
+SOURCES = ["vp_capi.hip", "vp_backward.hip", "vp_forward.hip", "vp_f64.hip", "vp_shift.hip"]
 ARCH = "gfx950"
 
 FLAGS = [

@@ -76,7 +76,6 @@ __device__ __forceinline__ void backproject_block(const OpArgs& g, const int log
     const int hb = bhc * VOX;
     const float xc = 0.5f * (float)(SX - 1);
     const float yc = 0.5f * (float)(SY - 1);
-    const float pc = 0.5f * (float)(PX - 1);
     const bool vec4 = (H & 3) == 0;
 
     // Footprint lane layout: voxel (lane % VPW) of this wave, angle lane / VPW.
@@ -92,16 +91,17 @@ __device__ __forceinline__ void backproject_block(const OpArgs& g, const int log
     auto tables = [&](int chunk, int slot) {
         if (tid < AC) {
             const int a = chunk * AC + tid;
-            const AngleK k = make_angle(angles[min(a, A - 1)], scale, inv_scale);
+            const int ac = min(a, A - 1);
+            const AngleK k = make_angle(angles[ac], g.du ? g.du[ac] : 0.0f, 0.5 * (double)(PX - 1), scale, inv_scale);
             s_k[slot][tid] = k;
             const float xa = (float)tx0 - xc;
             const float xb = (float)(min(tx0 + TX, SX) - 1) - xc;
             const float ya = (float)ty0 - yc;
             const float yb = (float)(min(ty0 + TY, SY) - 1) - yc;
-            int r = voxel_r(xa, ya, pc, k);
-            r = min(r, voxel_r(xb, ya, pc, k));
-            r = min(r, voxel_r(xa, yb, pc, k));
-            r = min(r, voxel_r(xb, yb, pc, k));
+            int r = voxel_r(xa, ya, k);
+            r = min(r, voxel_r(xb, ya, k));
+            r = min(r, voxel_r(xa, yb, k));
+            r = min(r, voxel_r(xb, yb, k));
             s_rlo[slot][tid] = r - 2;  // one spare pixel below r-1 for rounding
         }
     };
@@ -172,7 +172,7 @@ __device__ __forceinline__ void backproject_block(const OpArgs& g, const int log
             // Footprint of (voxel lv, angle ag + la) in this lane.
             const int al = ag + la;
             const AngleK k = s_k[slot][al];
-            const Taps t = voxel_taps(X, Y, pc, k);
+            const Taps t = voxel_taps(X, Y, k);
             const bool aval = c * AC + al < A;
             const int p0 = t.r - 1;
             const float wm = (aval && p0 >= 0 && p0 < PX) ? t.wm : 0.0f;
@@ -295,6 +295,7 @@ __device__ __forceinline__ OpArgs item_args(const BatchItem& it)
     g.PX = it.PX;
     g.accumulate = it.accumulate;
     const int bits = hi_bits(it.SY, it.SX, it.PX);
+    g.du = nullptr;
     g.scale = ldexp(1.0, bits);
     g.inv_scale = ldexp(1.0, -bits);
     return g;
@@ -386,6 +387,7 @@ hipError_t launch_backward_z0(const float* y, float* x, const float* angles, con
     a.accumulate = accumulate;
     a.scale = ldexp(1.0, g.bits);
     a.inv_scale = ldexp(1.0, -g.bits);
+    a.du = g.du;
     return default_variant(1, g.H) == 1 ? backward_single<1>(a, s) : backward_single<2>(a, s);
 }
 

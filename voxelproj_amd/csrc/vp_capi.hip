@@ -81,100 +81,264 @@ int validate_batch(int op, int variant, const vp_batch_item* items, int n)
 
 extern "C" {
 
-int vp_abi_version(void) { return 1; }
+int vp_abi_version(void) { return 2; }
 
 const char* vp_last_error(void) { return g_last_error.c_str(); }
 
-int vp_forward(int z_order, const float* x, float* y, const float* angles, int n_angles,
-               int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
+}  // extern "C"
+
+namespace {
+
+// Detector translation arguments: du / dv device arrays of n_angles floats
+// (or null); du_max bounds |du| (it sizes the exact hi/lo position split).
+int validate_shift(const float* du, double du_max, vp::Geom* g)
+{
+    if (!(du_max >= 0.0) || du_max > 1.0e6)
+        return fail(VP_ERR_ARG, "du_max must be a finite bound in [0, 1e6]");
+    g->du = du;
+    g->bits = vp::hi_bits(g->SY, g->SX, g->PX, du ? du_max : 0.0);
+    return VP_OK;
+}
+
+template <typename T>
+struct Workspace {
+    T* p = nullptr;
+    hipStream_t s;
+    hipError_t alloc(int64_t n, hipStream_t st)
+    {
+        s = st;
+        return hipMallocAsync((void**)&p, (size_t)n * sizeof(T), s);
+    }
+    hipError_t release()
+    {
+        hipError_t e = p ? hipFreeAsync(p, s) : hipSuccess;
+        p = nullptr;
+        return e;
+    }
+};
+
+// y += A x (float32), any layout, optional translations.
+int forward_f32(int z_order, const float* x, float* y, const float* angles, const vp::Geom& g,
+                const float* dv, hipStream_t s)
+{
+    const int64_t nx = g.SY * g.SX * g.H;
+    const int64_t ny = (int64_t)g.A * g.PX * g.H;
+    hipError_t e;
+    if (z_order == 0 && !dv) {
+        e = vp::launch_forward_z0(x, y, angles, g, 1, s);
+        return e == hipSuccess ? VP_OK : hip_fail(e, "forward launch");
+    }
+    // Workspace: the volume in slice-fastest order (z_order 2 only), the
+    // untranslated sinogram, and (z_order 2 with dv) the translated one.
+    const int64_t wx = z_order == 2 ? nx : 0;
+    const int64_t wy2 = (z_order == 2 && dv) ? ny : 0;
+    Workspace<float> ws;
+    e = ws.alloc(wx + ny + wy2, s);
+    if (e != hipSuccess) return hip_fail(e, "forward workspace");
+    const float* x0 = x;
+    if (z_order == 2) {
+        e = vp::launch_transpose(x, ws.p, g.H, g.SY * g.SX, 0, s);
+        x0 = ws.p;
+    }
+    float* y0 = ws.p + wx;
+    if (e == hipSuccess) e = vp::launch_forward_z0(x0, y0, angles, g, 0, s);
+    if (z_order == 0) {
+        if (e == hipSuccess)
+            e = vp::launch_vshift<float>(0, 0, y0, y, g.A, g.PX, g.H, 0, g.H, 0, dv, 1, s);
+    } else {
+        const float* yz = y0;
+        if (dv && e == hipSuccess) {
+            e = vp::launch_vshift<float>(0, 0, y0, y0 + ny, g.A, g.PX, g.H, 0, g.H, 0, dv, 0, s);
+            yz = y0 + ny;
+        }
+        if (e == hipSuccess) e = vp::launch_transpose(yz, y, (int64_t)g.A * g.PX, g.H, 1, s);
+    }
+    hipError_t ef = ws.release();
+    if (e != hipSuccess) return hip_fail(e, "forward launch");
+    if (ef != hipSuccess) return hip_fail(ef, "forward workspace release");
+    return VP_OK;
+}
+
+// x += AT y (float32), any layout, optional translations.
+int backward_f32(int z_order, const float* y, float* x, const float* angles, const vp::Geom& g,
+                 const float* dv, hipStream_t s)
+{
+    const int64_t nx = g.SY * g.SX * g.H;
+    const int64_t ny = (int64_t)g.A * g.PX * g.H;
+    hipError_t e;
+    if (z_order == 0 && !dv) {
+        e = vp::launch_backward_z0(y, x, angles, g, 1, s);
+        return e == hipSuccess ? VP_OK : hip_fail(e, "backward launch");
+    }
+    const int64_t wx = z_order == 2 ? nx : 0;
+    const int64_t wy = ny + ((z_order == 2 && dv) ? ny : 0);
+    Workspace<float> ws;
+    e = ws.alloc(wx + wy, s);
+    if (e != hipSuccess) return hip_fail(e, "backward workspace");
+    float* x0 = ws.p;
+    float* yb = ws.p + wx;
+    const float* yz = y;  // sinogram in slice-fastest order
+    if (z_order == 2) {
+        e = vp::launch_transpose(y, yb, g.H, (int64_t)g.A * g.PX, 0, s);
+        yz = yb;
+    }
+    if (dv && e == hipSuccess) {
+        float* g0 = (z_order == 2) ? yb + ny : yb;
+        e = vp::launch_vshift<float>(0, 1, yz, g0, g.A, g.PX, g.H, 0, g.H, 0, dv, 0, s);
+        yz = g0;
+    }
+    if (z_order == 0) {
+        if (e == hipSuccess) e = vp::launch_backward_z0(yz, x, angles, g, 1, s);
+    } else {
+        if (e == hipSuccess) e = vp::launch_backward_z0(yz, x0, angles, g, 0, s);
+        if (e == hipSuccess) e = vp::launch_transpose(x0, x, g.SY * g.SX, g.H, 1, s);
+    }
+    hipError_t ef = ws.release();
+    if (e != hipSuccess) return hip_fail(e, "backward launch");
+    if (ef != hipSuccess) return hip_fail(ef, "backward workspace release");
+    return VP_OK;
+}
+
+// y += A x (float64).  With dv the untranslated sinogram goes to a zeroed
+// workspace first (the FP64 kernels accumulate).
+int forward_f64(int z_order, const double* x, double* y, const float* angles, const vp::Geom& g,
+                const float* dv, hipStream_t s)
+{
+    hipError_t e;
+    if (!dv) {
+        e = vp::launch_forward_f64(z_order, x, y, angles, g, s);
+        return e == hipSuccess ? VP_OK : hip_fail(e, "forward f64 launch");
+    }
+    const int64_t ny = (int64_t)g.A * g.PX * g.H;
+    Workspace<double> ws;
+    e = ws.alloc(ny, s);
+    if (e != hipSuccess) return hip_fail(e, "forward f64 workspace");
+    e = vp::launch_fill<double>(ws.p, ny, 0.0, s);
+    if (e == hipSuccess) e = vp::launch_forward_f64(z_order, x, ws.p, angles, g, s);
+    if (e == hipSuccess)
+        e = vp::launch_vshift<double>(z_order, 0, ws.p, y, g.A, g.PX, g.H, 0, g.H, 0, dv, 1, s);
+    hipError_t ef = ws.release();
+    if (e != hipSuccess) return hip_fail(e, "forward f64 launch");
+    if (ef != hipSuccess) return hip_fail(ef, "forward f64 workspace release");
+    return VP_OK;
+}
+
+int backward_f64(int z_order, const double* y, double* x, const float* angles, const vp::Geom& g,
+                 const float* dv, hipStream_t s)
+{
+    hipError_t e;
+    if (!dv) {
+        e = vp::launch_backward_f64(z_order, y, x, angles, g, s);
+        return e == hipSuccess ? VP_OK : hip_fail(e, "backward f64 launch");
+    }
+    const int64_t ny = (int64_t)g.A * g.PX * g.H;
+    Workspace<double> ws;
+    e = ws.alloc(ny, s);
+    if (e != hipSuccess) return hip_fail(e, "backward f64 workspace");
+    e = vp::launch_vshift<double>(z_order, 1, y, ws.p, g.A, g.PX, g.H, 0, g.H, 0, dv, 0, s);
+    if (e == hipSuccess) e = vp::launch_backward_f64(z_order, ws.p, x, angles, g, s);
+    hipError_t ef = ws.release();
+    if (e != hipSuccess) return hip_fail(e, "backward f64 launch");
+    if (ef != hipSuccess) return hip_fail(ef, "backward f64 workspace release");
+    return VP_OK;
+}
+
+template <typename T, typename Op>
+int run_op(Op op, int z_order, const T* in, T* out, const float* angles, int n_angles, int64_t SY,
+           int64_t SX, int64_t H, int64_t PX, const float* du, const float* dv, double du_max,
+           void* stream)
 {
     g_last_error.clear();
     vp::Geom g;
     bool empty = false;
-    int rc = validate(z_order, x, y, angles, n_angles, SY, SX, H, PX, &g, &empty);
+    int rc = validate(z_order, in, out, angles, n_angles, SY, SX, H, PX, &g, &empty);
     if (rc != VP_OK || empty) return rc;
-    hipStream_t s = (hipStream_t)stream;
-    hipError_t e;
-    if (z_order == 0) {
-        e = vp::launch_forward_z0(x, y, angles, g, 1, s);
-        if (e != hipSuccess) return hip_fail(e, "forward z_order 0 launch");
-        return VP_OK;
-    }
-    // z_order 2: transpose the volume to slice-fastest order, project, and
-    // add the transposed sinogram back.
-    const int64_t nx = SY * SX * H;
-    const int64_t ny = (int64_t)n_angles * PX * H;
-    float* ws = nullptr;
-    e = hipMallocAsync((void**)&ws, (size_t)(nx + ny) * sizeof(float), s);
-    if (e != hipSuccess) return hip_fail(e, "forward z_order 2 workspace");
-    float* x0 = ws;
-    float* y0 = ws + nx;
-    e = vp::launch_transpose(x, x0, H, SY * SX, 0, s);
-    if (e == hipSuccess) e = vp::launch_forward_z0(x0, y0, angles, g, 0, s);
-    if (e == hipSuccess) e = vp::launch_transpose(y0, y, (int64_t)n_angles * PX, H, 1, s);
-    hipError_t ef = hipFreeAsync(ws, s);
-    if (e != hipSuccess) return hip_fail(e, "forward z_order 2 launch");
-    if (ef != hipSuccess) return hip_fail(ef, "forward z_order 2 workspace release");
-    return VP_OK;
+    rc = validate_shift(du, du_max, &g);
+    if (rc != VP_OK) return rc;
+    return op(z_order, in, out, angles, g, dv, (hipStream_t)stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int vp_forward(int z_order, const float* x, float* y, const float* angles, int n_angles,
+               int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
+{
+    return run_op<float>(forward_f32, z_order, x, y, angles, n_angles, SY, SX, H, PX, nullptr,
+                         nullptr, 0.0, stream);
 }
 
 int vp_backward(int z_order, const float* y, float* x, const float* angles, int n_angles,
                 int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
 {
-    g_last_error.clear();
-    vp::Geom g;
-    bool empty = false;
-    int rc = validate(z_order, y, x, angles, n_angles, SY, SX, H, PX, &g, &empty);
-    if (rc != VP_OK || empty) return rc;
-    hipStream_t s = (hipStream_t)stream;
-    hipError_t e;
-    if (z_order == 0) {
-        e = vp::launch_backward_z0(y, x, angles, g, 1, s);
-        if (e != hipSuccess) return hip_fail(e, "backward z_order 0 launch");
-        return VP_OK;
-    }
-    // z_order 2: transpose the sinogram to slice-fastest order, backproject,
-    // and add the transposed volume back.
-    const int64_t nx = SY * SX * H;
-    const int64_t ny = (int64_t)n_angles * PX * H;
-    float* ws = nullptr;
-    e = hipMallocAsync((void**)&ws, (size_t)(nx + ny) * sizeof(float), s);
-    if (e != hipSuccess) return hip_fail(e, "backward z_order 2 workspace");
-    float* x0 = ws;
-    float* y0 = ws + nx;
-    e = vp::launch_transpose(y, y0, H, (int64_t)n_angles * PX, 0, s);
-    if (e == hipSuccess) e = vp::launch_backward_z0(y0, x0, angles, g, 0, s);
-    if (e == hipSuccess) e = vp::launch_transpose(x0, x, SY * SX, H, 1, s);
-    hipError_t ef = hipFreeAsync(ws, s);
-    if (e != hipSuccess) return hip_fail(e, "backward z_order 2 launch");
-    if (ef != hipSuccess) return hip_fail(ef, "backward z_order 2 workspace release");
-    return VP_OK;
+    return run_op<float>(backward_f32, z_order, y, x, angles, n_angles, SY, SX, H, PX, nullptr,
+                         nullptr, 0.0, stream);
 }
 
 int vp_forward_f64(int z_order, const double* x, double* y, const float* angles, int n_angles,
                    int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
 {
-    g_last_error.clear();
-    vp::Geom g;
-    bool empty = false;
-    int rc = validate(z_order, x, y, angles, n_angles, SY, SX, H, PX, &g, &empty);
-    if (rc != VP_OK || empty) return rc;
-    hipError_t e = vp::launch_forward_f64(z_order, x, y, angles, g, (hipStream_t)stream);
-    if (e != hipSuccess) return hip_fail(e, "forward f64 launch");
-    return VP_OK;
+    return run_op<double>(forward_f64, z_order, x, y, angles, n_angles, SY, SX, H, PX, nullptr,
+                          nullptr, 0.0, stream);
 }
 
 int vp_backward_f64(int z_order, const double* y, double* x, const float* angles, int n_angles,
                     int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
 {
+    return run_op<double>(backward_f64, z_order, y, x, angles, n_angles, SY, SX, H, PX, nullptr,
+                          nullptr, 0.0, stream);
+}
+
+int vp_forward_geom(int z_order, const float* x, float* y, const float* angles, int n_angles,
+                    int64_t SY, int64_t SX, int64_t H, int64_t PX, const float* du,
+                    const float* dv, double du_max, void* stream)
+{
+    return run_op<float>(forward_f32, z_order, x, y, angles, n_angles, SY, SX, H, PX, du, dv,
+                         du_max, stream);
+}
+
+int vp_backward_geom(int z_order, const float* y, float* x, const float* angles, int n_angles,
+                     int64_t SY, int64_t SX, int64_t H, int64_t PX, const float* du,
+                     const float* dv, double du_max, void* stream)
+{
+    return run_op<float>(backward_f32, z_order, y, x, angles, n_angles, SY, SX, H, PX, du, dv,
+                         du_max, stream);
+}
+
+int vp_forward_geom_f64(int z_order, const double* x, double* y, const float* angles,
+                        int n_angles, int64_t SY, int64_t SX, int64_t H, int64_t PX,
+                        const float* du, const float* dv, double du_max, void* stream)
+{
+    return run_op<double>(forward_f64, z_order, x, y, angles, n_angles, SY, SX, H, PX, du, dv,
+                          du_max, stream);
+}
+
+int vp_backward_geom_f64(int z_order, const double* y, double* x, const float* angles,
+                         int n_angles, int64_t SY, int64_t SX, int64_t H, int64_t PX,
+                         const float* du, const float* dv, double du_max, void* stream)
+{
+    return run_op<double>(backward_f64, z_order, y, x, angles, n_angles, SY, SX, H, PX, du, dv,
+                          du_max, stream);
+}
+
+int vp_vshift(int z_order, int transpose, const float* in, float* out, int n_angles, int64_t PX,
+              int64_t H_in, int64_t in_offset, int64_t H_out, int64_t out_offset,
+              const float* dv, int accumulate, void* stream)
+{
     g_last_error.clear();
-    vp::Geom g;
-    bool empty = false;
-    int rc = validate(z_order, y, x, angles, n_angles, SY, SX, H, PX, &g, &empty);
-    if (rc != VP_OK || empty) return rc;
-    hipError_t e = vp::launch_backward_f64(z_order, y, x, angles, g, (hipStream_t)stream);
-    if (e != hipSuccess) return hip_fail(e, "backward f64 launch");
-    return VP_OK;
+    if (z_order != 0 && z_order != 2) return fail(VP_ERR_ARG, "z_order must be 0 or 2");
+    if (transpose != 0 && transpose != 1) return fail(VP_ERR_ARG, "transpose must be 0 or 1");
+    if (accumulate != 0 && accumulate != 1) return fail(VP_ERR_ARG, "accumulate must be 0 or 1");
+    if (n_angles < 0 || PX < 0 || H_in < 0 || H_out < 0)
+        return fail(VP_ERR_ARG, "extents must be non-negative");
+    if ((int64_t)n_angles * PX * H_out == 0) return VP_OK;
+    if (out == nullptr || dv == nullptr || (H_in > 0 && in == nullptr))
+        return fail(VP_ERR_ARG, "null device pointer");
+    hipError_t e = vp::launch_vshift<float>(z_order, transpose, in, out, n_angles, PX, H_in,
+                                            in_offset, H_out, out_offset, dv, accumulate,
+                                            (hipStream_t)stream);
+    return e == hipSuccess ? VP_OK : hip_fail(e, "vshift launch");
 }
 
 int vp_default_variant(int op, int64_t H) { return vp::default_variant(op, H); }

@@ -2,10 +2,11 @@
 //
 // Geometry (reference: voxelproj/csrc/include/sp_backward_z0.cuh:24-26,84 and
 // sp_forward_z0.cuh:132-156): voxel size = detector pixel size = 1, rotation
-// centre at the volume centre, detector centre on the rotation axis.  The
-// detector coordinate of voxel (ix, iy) at angle phi is
-//     u = (ix - xc) cos(phi) + (iy - yc) sin(phi) + pc,
-// xc = (SX-1)/2, yc = (SY-1)/2, pc = (PX-1)/2.  The weight of pixel p is the
+// centre at the volume centre, detector centre on the rotation axis (unless
+// translated).  The detector coordinate of voxel (ix, iy) at angle phi is
+//     u = (ix - xc) cos(phi) + (iy - yc) sin(phi) + pc - du,
+// xc = (SX-1)/2, yc = (SY-1)/2, pc = (PX-1)/2, du the detector translation
+// of that angle in pixels (0 in the reference geometry).  The weight of pixel p is the
 // area of the unit voxel inside the unit detector strip around p:
 //     W = F(p + 1/2 - u) - F(p - 1/2 - u),
 // F the CDF of the unit-area trapezoid with plateau half-width
@@ -19,7 +20,9 @@
 // B fractional bits and a small "lo" rest.  With B chosen from the volume
 // extents, (ix-xc)*c_hi + (iy-yc)*s_hi + pc is exact in FP32, and only the
 // tiny lo term and the final fractional offset round.  Positions are then
-// good to ~3e-8 and the weights to ~1e-7 absolute.
+// good to ~3e-8 and the weights to ~1e-7 absolute.  A translation du enters
+// the same way: its integer part joins pc in the exact hi sum, its fraction
+// (|.| <= 1/2) the lo sum.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -33,9 +36,12 @@ struct AngleK {
     float a, b;      // plateau / base half-widths of the footprint trapezoid
     float h;         // plateau height 1/max(|c|,|s|)
     float cd;        // h / (2 (b - a)); 0 when a == b (quadratic flanks vanish)
+    float pch, plo;  // pc - du = pch + plo: pch = pc - round(du) (exact), plo the fraction
 };
 
-__device__ __forceinline__ AngleK make_angle(float phi, double scale, double inv_scale)
+// pc = (PX-1)/2; du = detector translation of this angle (pixels).
+__device__ __forceinline__ AngleK make_angle(float phi, float du, double pc, double scale,
+                                             double inv_scale)
 {
     const double c = cos((double)phi);
     const double s = sin((double)phi);
@@ -55,6 +61,9 @@ __device__ __forceinline__ AngleK make_angle(float phi, double scale, double inv
     k.h = (float)h;
     const float bf = (float)b, af = (float)a;
     k.cd = (bf > af) ? 0.5f * k.h / (bf - af) : 0.0f;
+    const double dh = rint((double)du);
+    k.pch = (float)(pc - dh);
+    k.plo = (float)(dh - (double)du);
     return k;
 }
 
@@ -84,11 +93,11 @@ __device__ __forceinline__ float centre_pixel(float uhi, float ulo)
     return floorf((uhi + 0.5f) + ulo);
 }
 
-// X = ix - xc, Y = iy - yc (exact half-integers); pc = (PX-1)/2.
-__device__ __forceinline__ Taps voxel_taps(float X, float Y, float pc, const AngleK& k)
+// X = ix - xc, Y = iy - yc (exact half-integers).
+__device__ __forceinline__ Taps voxel_taps(float X, float Y, const AngleK& k)
 {
-    const float uhi = fmaf(X, k.chi, fmaf(Y, k.shi, pc));  // exact
-    const float ulo = fmaf(X, k.clo, Y * k.slo);           // small
+    const float uhi = fmaf(X, k.chi, fmaf(Y, k.shi, k.pch));  // exact
+    const float ulo = fmaf(X, k.clo, fmaf(Y, k.slo, k.plo));  // small
     const float rf = centre_pixel(uhi, ulo);
     const float t = (uhi - rf) + ulo;                      // u - r; uhi - rf exact
     Taps o;
@@ -123,19 +132,21 @@ __device__ __forceinline__ Taps2 anchor_left(const Taps& t)
 }
 
 // Centre pixel only, for window and segment bounds.
-__device__ __forceinline__ int voxel_r(float X, float Y, float pc, const AngleK& k)
+__device__ __forceinline__ int voxel_r(float X, float Y, const AngleK& k)
 {
-    const float uhi = fmaf(X, k.chi, fmaf(Y, k.shi, pc));
-    const float ulo = fmaf(X, k.clo, Y * k.slo);
+    const float uhi = fmaf(X, k.chi, fmaf(Y, k.shi, k.pch));
+    const float ulo = fmaf(X, k.clo, fmaf(Y, k.slo, k.plo));
     return (int)centre_pixel(uhi, ulo);
 }
 
-// Number of fractional bits of the hi split for a volume/detector extent.
-// Every partial sum of (ix-xc)*c_hi + (iy-yc)*s_hi + pc is a multiple of
-// 2^-(B+1) below 2^E in magnitude, so E + B + 1 <= 24 keeps it exact.
-__host__ __device__ inline int hi_bits(int64_t SY, int64_t SX, int64_t PX)
+// Number of fractional bits of the hi split for a volume/detector extent and
+// a bound on |du|.  Every partial sum of (ix-xc)*c_hi + (iy-yc)*s_hi + pch is
+// a multiple of 2^-(B+1) below 2^E in magnitude, so E + B + 1 <= 24 keeps it
+// exact.
+__host__ __device__ inline int hi_bits(int64_t SY, int64_t SX, int64_t PX, double du_max = 0.0)
 {
-    const double m = 0.5 * (double)(SX - 1) + 0.5 * (double)(SY - 1) + 0.5 * (double)(PX - 1) + 2.0;
+    const double m = 0.5 * (double)(SX - 1) + 0.5 * (double)(SY - 1) + 0.5 * (double)(PX - 1) +
+                     du_max + 2.0;
     int e = 0;
     while ((double)(1ll << e) < m) ++e;
     return 23 - e;

@@ -6,7 +6,8 @@
 //   A : thread per sinogram element, gathering the <= 5 voxels per line of the
 //       volume whose footprint can reach the pixel;
 //   AT: thread per volume element, gathering the 3 taps per angle.
-// Both layouts are handled through element strides; outputs accumulate.
+// Both layouts are handled through element strides; outputs accumulate.  A
+// per-angle detector translation du shifts every position u by -du.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -57,7 +58,8 @@ __device__ inline void angle64(float phi, double& c, double& s, double& a, doubl
 }
 
 __global__ void forward_f64_kernel(const double* __restrict__ x, double* __restrict__ y,
-                                   const float* __restrict__ angles, int A, int SY, int SX,
+                                   const float* __restrict__ angles,
+                                   const float* __restrict__ du, int A, int SY, int SX,
                                    int H, int PX, Strides st)
 {
     const int64_t n = (int64_t)A * PX * H;
@@ -68,7 +70,8 @@ __global__ void forward_f64_kernel(const double* __restrict__ x, double* __restr
     const int ia = (int)(i / ((int64_t)H * PX));
     double c, s, a, b, hh, cd;
     angle64(angles[ia], c, s, a, b, hh, cd);
-    const double xc = 0.5 * (SX - 1), yc = 0.5 * (SY - 1), pc = 0.5 * (PX - 1);
+    const double xc = 0.5 * (SX - 1), yc = 0.5 * (SY - 1);
+    const double pc = 0.5 * (PX - 1) - (du ? (double)du[ia] : 0.0);
     const bool modeX = fabs(c) >= fabs(s);
     const int nrows = modeX ? SY : SX;
     const int nalong = modeX ? SX : SY;
@@ -101,7 +104,8 @@ __global__ void forward_f64_kernel(const double* __restrict__ x, double* __restr
 }
 
 __global__ void backward_f64_kernel(const double* __restrict__ y, double* __restrict__ x,
-                                    const float* __restrict__ angles, int A, int SY, int SX,
+                                    const float* __restrict__ angles,
+                                    const float* __restrict__ du, int A, int SY, int SX,
                                     int H, int PX, Strides st)
 {
     const int64_t n = (int64_t)SY * SX * H;
@@ -110,11 +114,12 @@ __global__ void backward_f64_kernel(const double* __restrict__ y, double* __rest
     const int h = (int)(i % H);
     const int ix = (int)((i / H) % SX);
     const int iy = (int)(i / ((int64_t)H * SX));
-    const double xc = 0.5 * (SX - 1), yc = 0.5 * (SY - 1), pc = 0.5 * (PX - 1);
+    const double xc = 0.5 * (SX - 1), yc = 0.5 * (SY - 1);
     double acc = 0.0;
     for (int ia = 0; ia < A; ++ia) {
         double c, s, a, b, hh, cd;
         angle64(angles[ia], c, s, a, b, hh, cd);
+        const double pc = 0.5 * (PX - 1) - (du ? (double)du[ia] : 0.0);
         const Taps64 t = taps64(ix - xc, iy - yc, pc, c, s, a, b, hh, cd);
         const double w[3] = {t.wm, t.w0, t.wp};
         for (int k = 0; k < 3; ++k) {
@@ -145,7 +150,7 @@ hipError_t launch_forward_f64(int z_order, const double* x, double* y, const flo
 {
     const int64_t n = (int64_t)g.A * g.PX * g.H;
     const unsigned nb = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(forward_f64_kernel, dim3(nb), dim3(256), 0, s, x, y, angles, g.A,
+    hipLaunchKernelGGL(forward_f64_kernel, dim3(nb), dim3(256), 0, s, x, y, angles, g.du, g.A,
                        (int)g.SY, (int)g.SX, (int)g.H, (int)g.PX, strides(z_order, g));
     return hipGetLastError();
 }
@@ -155,7 +160,7 @@ hipError_t launch_backward_f64(int z_order, const double* y, double* x, const fl
 {
     const int64_t n = g.SY * g.SX * g.H;
     const unsigned nb = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(backward_f64_kernel, dim3(nb), dim3(256), 0, s, y, x, angles, g.A,
+    hipLaunchKernelGGL(backward_f64_kernel, dim3(nb), dim3(256), 0, s, y, x, angles, g.du, g.A,
                        (int)g.SY, (int)g.SX, (int)g.H, (int)g.PX, strides(z_order, g));
     return hipGetLastError();
 }

@@ -101,12 +101,12 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     const int ph = p0 + P;
     const float xc = 0.5f * (float)(SX - 1);
     const float yc = 0.5f * (float)(SY - 1);
-    const float pc = 0.5f * (float)(PX - 1);
     const bool vec4 = (H & 3) == 0;
 
     if (tid < NW) {
         const int at = ba * NW + tid;
-        const AngleK kk = make_angle(angles[min(at, A - 1)], scale, inv_scale);
+        const int ac = min(at, A - 1);
+        const AngleK kk = make_angle(angles[ac], g.du ? g.du[ac] : 0.0f, 0.5 * (double)(PX - 1), scale, inv_scale);
         s_k[tid] = kk;
         const float cc = kk.chi + kk.clo, ss = kk.shi + kk.slo;
         s_mode[tid] = (at < A) ? (fabsf(cc) >= fabsf(ss) ? 0 : 1) : -1;
@@ -115,6 +115,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     AngleK k = s_k[wave];
     k.chi = uni(k.chi); k.clo = uni(k.clo); k.shi = uni(k.shi); k.slo = uni(k.slo);
     k.a = uni(k.a); k.b = uni(k.b); k.h = uni(k.h); k.cd = uni(k.cd);
+    k.pch = uni(k.pch); k.plo = uni(k.plo);
     const int mode = uni(s_mode[wave]);
     // Lane w < NW describes angle w of the block.
     const AngleK kl = s_k[lane < NW ? lane : 0];
@@ -189,12 +190,13 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
         const float kp_l = px_ ? kcl : ksl;
         const float ot_l = px_ ? ksl : kcl;
         const float ik_l = 1.0f / kp_l;
+        const float pc_l = kl.pch + kl.plo;  // pc - du of angle w (segment bounds only)
 
         // Segment [jlo, jhi] of line R (centred row coordinate) for an angle
         // (kap, oth): voxels whose centre pixel can lie in [pl, ph], with one
         // pixel of margin each side.
-        auto segment = [&](float R, float kp, float ikp, float ot, int& jlo, int& jhi) {
-            const float ub = fmaf(-ca, kp, fmaf(R, ot, pc));
+        auto segment = [&](float R, float kp, float ikp, float ot, float pcw, int& jlo, int& jhi) {
+            const float ub = fmaf(-ca, kp, fmaf(R, ot, pcw));
             const float ja = ((float)pl - 1.5f - ub) * ikp;
             const float jb = ((float)ph + 1.5f - ub) * ikp;
             jlo = max(0, (int)floorf(fminf(ja, jb)) - 1);
@@ -217,7 +219,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
             for (int w = 0; w < NW; ++w) {
                 if (__builtin_amdgcn_readlane(model, w) == pass) {
                     int l, h;
-                    segment(R, rdlane(kp_l, w), rdlane(ik_l, w), rdlane(ot_l, w), l, h);
+                    segment(R, rdlane(kp_l, w), rdlane(ik_l, w), rdlane(ot_l, w), rdlane(pc_l, w), l, h);
                     if (h >= l) {
                         lo = min(lo, l);
                         hi = max(hi, h);
@@ -230,7 +232,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
             nonempty = __ballot(valid && hi >= lo);
             s_lo = 0;
             s_hi = -1;
-            if (mine) segment(R, kap, ikap, oth, s_lo, s_hi);
+            if (mine) segment(R, kap, ikap, oth, k.pch + k.plo, s_lo, s_hi);
         };
         // First piece at or after line `row` (pieces hold at most UMAX voxels).
         auto first_task = [&](int row) {
@@ -309,7 +311,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
             // footprint anchored at its first nonzero pixel l; slot q = l - (pl - 1)
             // covers l in [p0-2, p0+P-1], the voxels that touch the block's pixels
             const Taps2 tp = anchor_left(
-                voxel_taps(px_ ? (float)jm - xc : R, px_ ? R : (float)jm - yc, pc, k));
+                voxel_taps(px_ ? (float)jm - xc : R, px_ ? R : (float)jm - yc, k));
             const int lprev = __shfl_up(tp.l, 1);  // lane m-1 holds the previous voxel
             const bool valid = lane < n;
             const int l = valid ? tp.l : 0x3fffffff;
@@ -488,6 +490,7 @@ __device__ __forceinline__ OpArgs item_args(const BatchItem& it)
     g.PX = it.PX;
     g.accumulate = it.accumulate;
     const int bits = hi_bits(it.SY, it.SX, it.PX);
+    g.du = nullptr;
     g.scale = ldexp(1.0, bits);
     g.inv_scale = ldexp(1.0, -bits);
     return g;
@@ -601,6 +604,7 @@ hipError_t launch_forward_z0(const float* x, float* y, const float* angles, cons
     a.accumulate = accumulate;
     a.scale = ldexp(1.0, g.bits);
     a.inv_scale = ldexp(1.0, -g.bits);
+    a.du = g.du;
     switch (default_variant(0, g.H)) {
         case 1: return forward_single<1>(a, s);
         case 2: return forward_single<2>(a, s);

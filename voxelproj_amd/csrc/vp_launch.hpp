@@ -10,6 +10,7 @@ struct Geom {
     int64_t PX;         // detector pixels
     int A;              // number of angles (sinogram angle extent)
     int bits;           // fractional bits of the cos/sin hi split
+    const float* du = nullptr;  // per-angle detector translation (pixels), or null
 };
 
 // Arguments of one projector instance as the per-block device code sees them.
@@ -20,6 +21,7 @@ struct OpArgs {
     int A, SY, SX, H, PX;
     int accumulate;  // 1: out += op(in), 0: out = op(in)
     double scale, inv_scale;  // 2^B and 2^-B of the cos/sin hi split
+    const float* du;          // per-angle detector translation (pixels), or null
 };
 
 // One instance of a batched launch (layout shared with the C ABI's vp_batch_item).
@@ -59,6 +61,21 @@ hipError_t launch_forward_f64(int z_order, const double* x, double* y, const flo
                               const Geom& g, hipStream_t s);
 hipError_t launch_backward_f64(int z_order, const double* y, double* x, const float* angles,
                                const Geom& g, hipStream_t s);
+
+// Linear interpolation along H of a sinogram, per angle (detector translation
+// dv along H, in slices).  Forward (transpose = 0): out at global slice i
+// samples in at global position i + dv[a]; transpose = 1 applies the adjoint.
+// in / out hold slices [in_off, in_off + Hin) / [out_off, out_off + Hout) of
+// the full sinogram; positions outside the input slab read 0.  z_order 0:
+// (A, PX, H); 2: (H, A, PX).  Outputs accumulate when accumulate != 0.
+template <typename T>
+hipError_t launch_vshift(int z_order, int transpose, const T* in, T* out, int A, int64_t PX,
+                         int64_t Hin, int64_t in_off, int64_t Hout, int64_t out_off,
+                         const float* dv, int accumulate, hipStream_t s);
+
+// p[0..n) = v (stream-ordered fill kernel; usable under graph capture).
+template <typename T>
+hipError_t launch_fill(T* p, int64_t n, T v, hipStream_t s);
 
 // out[c][r] (+)= in[r][c] for an R x C row-major matrix.
 hipError_t launch_transpose(const float* in, float* out, int64_t R, int64_t C, int accumulate,

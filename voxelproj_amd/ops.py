@@ -10,7 +10,15 @@ stream-ordered on the current stream.
 Layouts (voxelproj/csrc/voxelproj.cpp:15-33):
   z_order 0: x (SY, SX, H), y (A, PX, H)
   z_order 2: x (H, SY, SX), y (H, A, PX)
+
+Detector translations (an extension; the reference lists them as future work,
+README.md:65-68): ``du`` shifts the detector along its pixel axis (pixels),
+``dv`` along H (slices, linear interpolation between slices, zero outside the
+volume).  Each is None, one number, or one value per angle.  The pair stays
+exactly adjoint.  ``volume_offset_du`` converts an in-plane volume translation
+into the equivalent per-angle ``du``.
 """
+import numpy as np
 import torch
 
 from . import _lib
@@ -21,6 +29,48 @@ _ENTRY = {
     (torch.float64, "A"): "vp_forward_f64",
     (torch.float64, "AT"): "vp_backward_f64",
 }
+_ENTRY_GEOM = {
+    (torch.float32, "A"): "vp_forward_geom",
+    (torch.float32, "AT"): "vp_backward_geom",
+    (torch.float64, "A"): "vp_forward_geom_f64",
+    (torch.float64, "AT"): "vp_backward_geom_f64",
+}
+
+
+class Shift:
+    """A per-angle detector translation on the device, with max |value|."""
+
+    def __init__(self, values, n, device, name):
+        if isinstance(values, torch.Tensor):
+            t = values.detach().to(dtype=torch.float32).reshape(-1)
+        else:
+            t = torch.as_tensor(np.asarray(values, dtype=np.float32).reshape(-1))
+        if t.numel() == 1:
+            t = t.expand(n)
+        if t.numel() != n:
+            raise ValueError(f"{name} needs 1 or {n} values, got {t.numel()}")
+        host = t.cpu()
+        if not bool(torch.isfinite(host).all()):
+            raise ValueError(f"{name} must be finite")
+        self.max_abs = float(host.abs().max()) if n else 0.0
+        if self.max_abs > 1.0e6:
+            raise ValueError(f"|{name}| must be <= 1e6")
+        self.tensor = t.to(device=device).contiguous()
+
+
+def make_shifts(du, dv, n, device):
+    """(Shift or None, Shift or None) for the given translations."""
+    su = None if du is None else (du if isinstance(du, Shift) else Shift(du, n, device, "du"))
+    sv = None if dv is None else (dv if isinstance(dv, Shift) else Shift(dv, n, device, "dv"))
+    return su, sv
+
+
+def volume_offset_du(angles, dx=0.0, dy=0.0):
+    """Per-angle du equivalent to translating the volume by (dx, dy) voxels
+    in its plane: the content moves by dx cos + dy sin on the detector, i.e.
+    du = -(dx cos(phi) + dy sin(phi))."""
+    a = np.asarray(angles.cpu() if isinstance(angles, torch.Tensor) else angles, dtype=np.float64)
+    return -(dx * np.cos(a) + dy * np.sin(a))
 
 
 def vol_dims(shape, z_order):
@@ -69,18 +119,27 @@ def _check(t, name):
         raise RuntimeError(f"{name} must be contiguous")
 
 
-def _launch(kind, z_order, inp, out, angles, A, SY, SX, H, PX):
+def _launch(kind, z_order, inp, out, angles, A, SY, SX, H, PX, du=None, dv=None):
     lib = _lib.load()
-    name = _ENTRY[(inp.dtype, kind)]
     with torch.cuda.device(inp.device):
         stream = torch.cuda.current_stream(inp.device).cuda_stream
-        rc = getattr(lib, name)(z_order, inp.data_ptr(), out.data_ptr(), angles.data_ptr(), A,
-                                SY, SX, H, PX, stream)
+        if du is None and dv is None:
+            name = _ENTRY[(inp.dtype, kind)]
+            rc = getattr(lib, name)(z_order, inp.data_ptr(), out.data_ptr(), angles.data_ptr(),
+                                    A, SY, SX, H, PX, stream)
+        else:
+            name = _ENTRY_GEOM[(inp.dtype, kind)]
+            rc = getattr(lib, name)(z_order, inp.data_ptr(), out.data_ptr(), angles.data_ptr(),
+                                    A, SY, SX, H, PX,
+                                    du.tensor.data_ptr() if du is not None else None,
+                                    dv.tensor.data_ptr() if dv is not None else None,
+                                    du.max_abs if du is not None else 0.0, stream)
     _lib.check(rc, name)
 
 
-def project(x, angles, PX=None, z_order=0, out=None):
-    """y = A x (or out += A x).  x: device tensor, float32 or float64."""
+def project(x, angles, PX=None, z_order=0, out=None, du=None, dv=None):
+    """y = A x (or out += A x).  x: device tensor, float32 or float64;
+    du / dv: optional detector translations (see the module docstring)."""
     _check(x, "x")
     SY, SX, H = vol_dims(x.shape, z_order)
     a = device_angles(angles, x.device)
@@ -92,12 +151,14 @@ def project(x, angles, PX=None, z_order=0, out=None):
         _check(out, "out")
         if tuple(out.shape) != shape or out.dtype != x.dtype:
             raise RuntimeError(f"out must be {x.dtype} of shape {shape}")
-    _launch("A", z_order, x, out, a, a.numel(), SY, SX, H, PX)
+    su, sv = make_shifts(du, dv, a.numel(), x.device)
+    _launch("A", z_order, x, out, a, a.numel(), SY, SX, H, PX, su, sv)
     return out
 
 
-def backproject(y, angles, SY=None, SX=None, z_order=0, out=None):
-    """x = AT y (or out += AT y).  y: device tensor, float32 or float64."""
+def backproject(y, angles, SY=None, SX=None, z_order=0, out=None, du=None, dv=None):
+    """x = AT y (or out += AT y).  y: device tensor, float32 or float64;
+    du / dv: optional detector translations (see the module docstring)."""
     _check(y, "y")
     A, PX, H = sino_dims(y.shape, z_order)
     a = device_angles(angles, y.device)
@@ -112,5 +173,6 @@ def backproject(y, angles, SY=None, SX=None, z_order=0, out=None):
         _check(out, "out")
         if tuple(out.shape) != shape or out.dtype != y.dtype:
             raise RuntimeError(f"out must be {y.dtype} of shape {shape}")
-    _launch("AT", z_order, y, out, a, A, SY, SX, H, PX)
+    su, sv = make_shifts(du, dv, A, y.device)
+    _launch("AT", z_order, y, out, a, A, SY, SX, H, PX, su, sv)
     return out

@@ -52,18 +52,20 @@ class SIRT:
     """
 
     def __init__(self, angles, SY, SX, H, PX=None, z_order=0, device="cuda",
-                 dtype=torch.float32):
+                 dtype=torch.float32, du=None, dv=None):
         self.z = z_order
         self.SY, self.SX, self.H = SY, SX, H
         self.PX = max(SY, SX) if PX is None else PX
         self.angles = ops.device_angles(angles, device)
         A = self.angles.numel()
+        # optional detector translations, fixed for the solver's lifetime
+        self.du, self.dv = ops.make_shifts(du, dv, A, self.angles.device)
         self.vshape = ops.vol_shape(SY, SX, H, z_order)
         self.sshape = ops.sino_shape(A, self.PX, H, z_order)
         ones_v = torch.ones(self.vshape, device=device, dtype=dtype)
         ones_s = torch.ones(self.sshape, device=device, dtype=dtype)
-        rs = ops.project(ones_v, self.angles, self.PX, z_order)
-        cs = ops.backproject(ones_s, self.angles, SY, SX, z_order)
+        rs = ops.project(ones_v, self.angles, self.PX, z_order, du=self.du, dv=self.dv)
+        cs = ops.backproject(ones_s, self.angles, SY, SX, z_order, du=self.du, dv=self.dv)
         self.R = rs.clamp_min(1e-6).reciprocal_().mul_(rs > 1e-6)
         self.C = cs.clamp_min(1e-6).reciprocal_().mul_(cs > 1e-6)
         self._ax = torch.empty(self.sshape, device=device, dtype=dtype)
@@ -73,11 +75,12 @@ class SIRT:
         """Run n_iter SIRT updates on x (in place) for data b; returns x."""
         for _ in range(n_iter):
             self._ax.zero_()
-            ops.project(x, self.angles, self.PX, self.z, out=self._ax)  # A x
+            ops.project(x, self.angles, self.PX, self.z, out=self._ax, du=self.du, dv=self.dv)
             torch.sub(b, self._ax, out=self._ax)                         # b - A x
             self._ax.mul_(self.R)
             self._bx.zero_()
-            ops.backproject(self._ax, self.angles, self.SY, self.SX, self.z, out=self._bx)
+            ops.backproject(self._ax, self.angles, self.SY, self.SX, self.z, out=self._bx,
+                            du=self.du, dv=self.dv)
             x.addcmul_(self.C, self._bx)
             if nonneg:
                 x.clamp_(min=0)
@@ -85,7 +88,7 @@ class SIRT:
 
     def residual(self, x, b):
         self._ax.zero_()
-        ops.project(x, self.angles, self.PX, self.z, out=self._ax)
+        ops.project(x, self.angles, self.PX, self.z, out=self._ax, du=self.du, dv=self.dv)
         return float(torch.linalg.vector_norm((self._ax - b).double()))
 
 

@@ -28,11 +28,15 @@ error or into the intended result):
 * any number of angles works (the reference's AT is limited to 512).
 
 ``block_size`` is accepted and ignored (no meaning on CDNA).
+
+Extension (keyword-only, absent from the reference): ``du`` / ``dv`` detector
+translations along the pixel axis / along H, each None, a number or one value
+per angle (see ``voxelproj_amd.ops``).
 """
 import numpy as np
 import torch
 
-from . import _CU
+from . import _CU, ops
 
 
 class ZOrderError(KeyError, ValueError):
@@ -142,6 +146,10 @@ def _finish(out_obj, out_dev, in_obj, is_numpy, device):
     return out_dev.to(device) if device is not None else out_dev
 
 
+def _sino_px(y_shape, z_order):
+    return y_shape[1] if z_order == 0 else y_shape[2]
+
+
 def _angles_dev(angles):
     if isinstance(angles, torch.Tensor):
         a = angles.to(dtype=torch.float32, device="cuda")
@@ -150,7 +158,7 @@ def _angles_dev(angles):
     return a.reshape(-1).contiguous()
 
 
-def forward(x, angles, y, z_order=0, block_size=[8, 8, 8]):  # noqa: B006
+def forward(x, angles, y, z_order=0, block_size=[8, 8, 8], *, du=None, dv=None):  # noqa: B006
     """Forward projection y (+)= A x.  See the module docstring for semantics."""
     _check_zorder(z_order)
     _check_container(y, "y")
@@ -166,11 +174,15 @@ def forward(x, angles, y, z_order=0, block_size=[8, 8, 8]):  # noqa: B006
     x_dev = gpu_array_with_default_shape(x)
     a_dev = _angles_dev(angles)
     y_dev, _ = _device_out(y, y_shape)
-    _FORWARD[z_order](x_dev, y_dev, a_dev, block_size=block_size)
+    if du is None and dv is None:
+        _FORWARD[z_order](x_dev, y_dev, a_dev, block_size=block_size)
+    else:
+        ops.project(x_dev.contiguous(), a_dev, _sino_px(y_shape, z_order), z_order, out=y_dev,
+                    du=du, dv=dv)
     return _finish(y, y_dev, x, is_numpy, device)
 
 
-def backward(y, angles, x, z_order=0, block_size=[8, 8, 8]):  # noqa: B006
+def backward(y, angles, x, z_order=0, block_size=[8, 8, 8], *, du=None, dv=None):  # noqa: B006
     """Backprojection x (+)= AT y.  See the module docstring for semantics."""
     _check_zorder(z_order)
     _check_container(x, "x")
@@ -186,5 +198,9 @@ def backward(y, angles, x, z_order=0, block_size=[8, 8, 8]):  # noqa: B006
     y_dev = gpu_array_with_default_shape(y)
     a_dev = _angles_dev(angles)
     x_dev, _ = _device_out(x, x_shape)
-    _BACKWARD[z_order](y_dev, x_dev, a_dev, block_size=block_size)
+    if du is None and dv is None:
+        _BACKWARD[z_order](y_dev, x_dev, a_dev, block_size=block_size)
+    else:
+        SY, SX, _ = ops.vol_dims(x_shape, z_order)
+        ops.backproject(y_dev.contiguous(), a_dev, SY, SX, z_order, out=x_dev, du=du, dv=dv)
     return _finish(x, x_dev, y, is_numpy, device)
