@@ -62,8 +62,14 @@ __device__ __forceinline__ void backproject_block(const OpArgs& g, const int log
     static_assert(1024 % (VOX * 4) == 0 && (AC * WIN) % (1024 / (VOX * 4)) == 0,
                   "one 16-byte-per-lane DMA instruction must cover whole pixel rows");
     __shared__ float4 s_y4[2][STAGE / 4];
-    __shared__ AngleK s_k[3][AC];
-    __shared__ int s_rlo[3][AC];
+    // Angle constants and window bases for a range of kTabAngles angles,
+    // double-buffered (the next range is built while the current one is in
+    // use); every thread builds at most one angle, so the FP64 angle math of
+    // a range costs one latency and the chunk loop reads only LDS.
+    constexpr int kTabAngles = 256;
+    static_assert(kTabAngles % AC == 0 && kTabAngles <= 64 * NW, "angle table range");
+    __shared__ AngleK s_k[2][kTabAngles];
+    __shared__ int s_rlo[2][kTabAngles];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -87,13 +93,13 @@ __device__ __forceinline__ void backproject_block(const OpArgs& g, const int log
 
     const int nchunks = (A + AC - 1) / AC;
 
-    // Per-chunk angle constants and window base, written by threads < AC.
-    auto tables = [&](int chunk, int slot) {
-        if (tid < AC) {
-            const int a = chunk * AC + tid;
-            const int ac = min(a, A - 1);
-            const AngleK k = make_angle(angles[ac], g.du ? g.du[ac] : 0.0f, 0.5 * (double)(PX - 1), scale, inv_scale);
-            s_k[slot][tid] = k;
+    // Angle constants and window base of angles [a0, a0 + kTabAngles) into
+    // table buffer tb (one angle per thread).
+    auto tables = [&](int a0, int tb) {
+        if (tid < kTabAngles && a0 + tid < A) {
+            const int a = a0 + tid;
+            const AngleK k = make_angle(angles[a], g.du ? g.du[a] : 0.0f, 0.5 * (double)(PX - 1), scale, inv_scale);
+            s_k[tb][tid] = k;
             const float xa = (float)tx0 - xc;
             const float xb = (float)(min(tx0 + TX, SX) - 1) - xc;
             const float ya = (float)ty0 - yc;
@@ -102,12 +108,15 @@ __device__ __forceinline__ void backproject_block(const OpArgs& g, const int log
             r = min(r, voxel_r(xb, ya, k));
             r = min(r, voxel_r(xa, yb, k));
             r = min(r, voxel_r(xb, yb, k));
-            s_rlo[slot][tid] = r - 2;  // one spare pixel below r-1 for rounding
+            s_rlo[tb][tid] = r - 2;  // one spare pixel below r-1 for rounding
         }
     };
+    // Table buffer and entry of angle a (a < A).
+    auto tab_of = [&](int a) { return (a / kTabAngles) & 1; };
+    auto ent_of = [&](int a) { return a % kTabAngles; };
     // DMA of a chunk's windows: pixel rows [rlo, rlo+WIN) of each angle,
     // clamped to the detector (out-of-detector taps carry zero weight).
-    auto fetch = [&](int chunk, int slot, int b) {
+    auto fetch = [&](int chunk, int b) {
         float* dst0 = reinterpret_cast<float*>(s_y4[b]);
         if (vec4) {
             constexpr int RPI = 1024 / (VOX * 4);  // pixel rows per DMA instruction
@@ -120,7 +129,7 @@ __device__ __forceinline__ void backproject_block(const OpArgs& g, const int log
                 const int a = row / WIN;
                 const int kk = row % WIN;
                 const int ag = min(chunk * AC + a, A - 1);
-                const int p = min(max(s_rlo[slot][a] + kk, 0), PX - 1);
+                const int p = min(max(s_rlo[tab_of(ag)][ent_of(ag)] + kk, 0), PX - 1);
                 __builtin_amdgcn_global_load_lds(
                     (const void*)(y + ((int64_t)ag * PX + p) * H + hoff),
                     (__attribute__((address_space(3))) void*)(dst0 + ins * RPI * VOX), 16, 0, 0);
@@ -132,7 +141,7 @@ __device__ __forceinline__ void backproject_block(const OpArgs& g, const int log
                 const int a = row / WIN;
                 const int kk = row % WIN;
                 const int ag = min(chunk * AC + a, A - 1);
-                const int p = min(max(s_rlo[slot][a] + kk, 0), PX - 1);
+                const int p = min(max(s_rlo[tab_of(ag)][ent_of(ag)] + kk, 0), PX - 1);
                 const int hh = min(hb + 64 * c + lane, H - 1);
                 __builtin_amdgcn_global_load_lds(
                     (const void*)(y + ((int64_t)ag * PX + p) * H + hh),
@@ -148,17 +157,25 @@ __device__ __forceinline__ void backproject_block(const OpArgs& g, const int log
         for (int j = 0; j < HL; ++j) acc[v][j] = 0.0f;
 
     tables(0, 0);
-    if (nchunks > 1) tables(1, 1);
     __syncthreads();
-    fetch(0, 0, 0);
+    fetch(0, 0);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 
     for (int c = 0; c < nchunks; ++c) {
         const int b = c & 1;
-        const int slot = c % 3;
-        if (c + 1 < nchunks) fetch(c + 1, (c + 1) % 3, b ^ 1);
-        if (c + 2 < nchunks) tables(c + 2, (c + 2) % 3);
+        const int a0 = c * AC;  // first angle of the chunk
+        const int tb = tab_of(a0);
+        if (c + 1 < nchunks) {
+            const int an = a0 + AC;
+            if (an % kTabAngles == 0) {
+                // the next chunk starts a new angle range: build its table
+                // (into the other buffer, idle since the previous range)
+                tables(an, tab_of(an));
+                __syncthreads();
+            }
+            fetch(c + 1, b ^ 1);
+        }
 
         float part[VPW][HL];
 #pragma unroll
@@ -171,14 +188,15 @@ __device__ __forceinline__ void backproject_block(const OpArgs& g, const int log
         for (int ag = 0; ag < AC; ag += APL) {
             // Footprint of (voxel lv, angle ag + la) in this lane.
             const int al = ag + la;
-            const AngleK k = s_k[slot][al];
+            const int ae = ent_of(min(a0 + al, A - 1));  // same range: a0 <= A - 1
+            const AngleK k = s_k[tb][ae];
             const Taps t = voxel_taps(X, Y, k);
             const bool aval = c * AC + al < A;
             const int p0 = t.r - 1;
             const float wm = (aval && p0 >= 0 && p0 < PX) ? t.wm : 0.0f;
             const float w0 = (aval && p0 + 1 >= 0 && p0 + 1 < PX) ? t.w0 : 0.0f;
             const float wp = (aval && p0 + 2 >= 0 && p0 + 2 < PX) ? t.wp : 0.0f;
-            int off = p0 - s_rlo[slot][al];
+            int off = p0 - s_rlo[tb][ae];
             off = min(max(off, 0), WIN - 3);
             const int foff = (al * WIN + off) * VOX;  // LDS float offset of pixel r-1
 #pragma unroll
