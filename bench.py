@@ -56,15 +56,20 @@ def count_nnz(angles, SY, SX, PX):
     return total
 
 
-def time_op(fn, args, n, stream):
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for _ in range(n):
+def time_op(fn, args, n, stream, warmup=3):
+    """Per-call launch durations (ms) of n back-to-back calls, each bracketed
+    by HIP events on the kernels' stream, after `warmup` untimed calls.
+    Returns (median, p10, p90)."""
+    for _ in range(warmup):
         fn(*args)
-    end.record(stream)
-    end.synchronize()
-    return start.elapsed_time(end) / n
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(n + 1)]
+    ev[0].record(stream)
+    for i in range(n):
+        fn(*args)
+        ev[i + 1].record(stream)
+    ev[-1].synchronize()
+    t = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(n)])
+    return float(np.median(t)), float(np.percentile(t, 10)), float(np.percentile(t, 90))
 
 
 def cpu_baseline(N, A, H_sample, budget_s=10.0):
@@ -124,15 +129,13 @@ def bench_batched(dev, n=48, seed=0, reps=3):
             b.zero_()
             ops.backproject(y, a, x.shape[0], x.shape[1], out=b)
 
-    plain()
-    ms_plain = time_op(plain, (), reps, stream)
+    ms_plain = time_op(plain, (), reps, stream, warmup=1)[0]
     pa = batch.BatchPlan("A", xs, ys, angs)
     pb = batch.BatchPlan("AT", ys, xb, angs)
     tune_a = batch.autotune(pa, persist=False)
     tune_b = batch.autotune(pb, persist=False)
     graph = batch.capture_pair(pa, pb)
-    graph.replay()
-    ms_batch = time_op(graph.replay, (), reps, stream)
+    ms_batch = time_op(graph.replay, (), reps, stream, warmup=1)[0]
     return {
         "workload": f"{n} volumes, SX=SY in {{96..384}}, H in {{32..256}}, "
                     f"n_angles in {{60..1024}}, A then AT (z_order 0)",
@@ -242,13 +245,15 @@ def main():
     vox_ang = float(N) * N * N * A
     value = 2.0 * vox_ang / (ms_step * 1e-3) / 1e9
 
-    # Per-kernel launch durations (HIP events on the kernels' stream).
-    n_t = max(3, args.steps)
-    ms_a = time_op(_CU.forward_z0, (x, y, ang), n_t, stream)
-    ms_at = time_op(_CU.backward_z0, (yb, xb, ang), n_t, stream)
-    per_op = {"A_z0_ms": round(ms_a, 4), "AT_z0_ms": round(ms_at, 4),
+    # Per-kernel launch durations (HIP events on the kernels' stream): median
+    # of 20 calls after 3 warm-ups, with p10 / p90.
+    n_t = 20
+    ms_a, a10, a90 = time_op(_CU.forward_z0, (x, y, ang), n_t, stream)
+    ms_at, t10, t90 = time_op(_CU.backward_z0, (yb, xb, ang), n_t, stream)
+    per_op = {"A_z0_ms": round(ms_a, 4), "A_z0_ms_p10_p90": [round(a10, 4), round(a90, 4)],
+              "AT_z0_ms": round(ms_at, 4), "AT_z0_ms_p10_p90": [round(t10, 4), round(t90, 4)],
               "A_z0_gvox_ang_s": round(vox_ang / ms_a / 1e6, 1),
-              "AT_z0_gvox_ang_s": round(vox_ang / ms_at / 1e6, 1)}
+              "AT_z0_gvox_ang_s": round(vox_ang / ms_at / 1e6, 1), "timed_calls": n_t}
     if not args.no_z2:
         del y, xb
         x2 = x.permute(2, 0, 1).contiguous()
@@ -257,8 +262,8 @@ def main():
         yb2 = yb.permute(2, 0, 1).contiguous()
         del yb
         xb2 = torch.zeros((N, N, N), device=dev)
-        ms_a2 = time_op(_CU.forward_z2, (x2, y2, ang), n_t, stream)
-        ms_at2 = time_op(_CU.backward_z2, (yb2, xb2, ang), n_t, stream)
+        ms_a2 = time_op(_CU.forward_z2, (x2, y2, ang), n_t, stream)[0]
+        ms_at2 = time_op(_CU.backward_z2, (yb2, xb2, ang), n_t, stream)[0]
         per_op.update({"A_z2_ms": round(ms_a2, 4), "AT_z2_ms": round(ms_at2, 4),
                        "A_z2_gvox_ang_s": round(vox_ang / ms_a2 / 1e6, 1),
                        "AT_z2_gvox_ang_s": round(vox_ang / ms_at2 / 1e6, 1)})
