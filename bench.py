@@ -196,8 +196,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the projector path is not sharded across GPUs "
-              "(see DESIGN.md for how it would shard along H). Run with --gpus 1.",
-              file=sys.stderr)
+              "(it would shard along H, where slices are independent and need no exchange). "
+              "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import voxelproj_amd as vp

@@ -1,5 +1,5 @@
 """GPU parity of the translated detector geometry (du along the detector, dv
-along H) and of the H-chunked host-resident driver.
+along H) and of the H-slab streaming of host-resident data (chunked.py).
 
 The translations are an extension (the reference lists them as future work,
 README.md:65-68); the checker is the FP64 geometry oracle

@@ -8,7 +8,7 @@
 //     g[a, p, j] = (1 - f) y[a, p, j - k] + f y[a, p, j - k - 1].
 // Both are gathers (one thread per output element), so no atomics.  Slabs:
 // `in` holds global slices [in_off, in_off + Hin), `out` [out_off, out_off +
-// Hout), which lets the H-chunked driver process a slab with its halo.
+// Hout), which lets the H-slab streaming path process a slab with its halo.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
