@@ -130,10 +130,11 @@ def bench_batched(dev, n=48, seed=0, reps=3):
             ops.backproject(y, a, x.shape[0], x.shape[1], out=b)
 
     ms_plain = time_op(plain, (), reps, stream, warmup=1)[0]
-    pa = batch.BatchPlan("A", xs, ys, angs)
-    pb = batch.BatchPlan("AT", ys, xb, angs)
-    tune_a = batch.autotune(pa, persist=False)
-    tune_b = batch.autotune(pb, persist=False)
+    # one persistent launch per kernel-variant group, each group autotuned
+    pa = batch.GroupedPlan("A", xs, ys, angs)
+    pb = batch.GroupedPlan("AT", ys, xb, angs)
+    tune_a = batch.autotune_grouped(pa, persist=False)
+    tune_b = batch.autotune_grouped(pb, persist=False)
     graph = batch.capture_pair(pa, pb)
     ms_batch = time_op(graph.replay, (), reps, stream, warmup=1)[0]
     return {
@@ -144,9 +145,10 @@ def bench_batched(dev, n=48, seed=0, reps=3):
         "plain_gvox_ang_s": round(work / ms_plain / 1e6, 1),
         "batched_gvox_ang_s": round(work / ms_batch / 1e6, 1),
         "speedup": round(ms_plain / ms_batch, 3),
-        "variants": {"A": pa.variant, "AT": pb.variant},
-        "autotune_ms": {"A": {str(k): round(v, 3) for k, v in tune_a.items()},
-                        "AT": {str(k): round(v, 3) for k, v in tune_b.items()}},
+        "groups": {"A": {str(v): n for v, n in pa.variants.items()},
+                   "AT": {str(v): n for v, n in pb.variants.items()}},
+        "autotune_ms": {"A": [{str(k): round(v, 3) for k, v in t.items()} for _, t in tune_a],
+                        "AT": [{str(k): round(v, 3) for k, v in t.items()} for _, t in tune_b]},
     }
 
 

@@ -191,6 +191,52 @@ def autotune(plan, reps=3, persist=True):
     return times
 
 
+class GroupedPlan:
+    """A batch split into groups by kernel variant, one persistent launch per
+    group.  One variant for a whole uneven batch leaves lanes idle on the
+    instances whose H it does not suit (4 slices per lane on H = 32 leaves 7
+    lanes in 8 idle); grouping gives every instance the variant its H calls
+    for -- the one the single-instance operator uses, so each result equals
+    the plain operator's bit for bit -- while each group still runs as one
+    persistent launch.  ``variant_of(H)`` overrides the grouping rule.
+    """
+
+    def __init__(self, op, ins, outs, angles, accumulate=False, grid=None, variant_of=None):
+        if not (len(ins) == len(outs) == len(angles)) or len(ins) == 0:
+            raise ValueError("ins, outs and angles must be non-empty lists of equal length")
+        lib = _lib.load()
+        code = 0 if op == "A" else 1
+        groups = {}
+        for i, t in enumerate(ins):
+            H = int(t.shape[2])  # z_order 0: H is the last extent of volumes and sinograms
+            v = int(variant_of(H)) if variant_of is not None else int(lib.vp_default_variant(code, H))
+            groups.setdefault(v, []).append(i)
+        self.op = op
+        self.device = ins[0].device
+        self.plans = [BatchPlan(op, [ins[i] for i in idx], [outs[i] for i in idx],
+                                [angles[i] for i in idx], accumulate, v, grid)
+                      for v, idx in sorted(groups.items())]
+        self.outs = list(outs)
+
+    @property
+    def work(self):
+        return sum(p.work for p in self.plans)
+
+    @property
+    def variants(self):
+        return {p.variant: p.n for p in self.plans}
+
+    def launch(self):
+        for p in self.plans:
+            p.launch()
+        return self.outs
+
+
+def autotune_grouped(gplan, reps=3, persist=True):
+    """Autotune every group of a GroupedPlan; returns [(group size, {variant: ms})]."""
+    return [(p.n, autotune(p, reps, persist)) for p in gplan.plans]
+
+
 # --------------------------------------------------------------- graphs
 def capture_pair(plan_a, plan_at, warmup=1):
     """Capture `plan_a.launch()` followed by `plan_at.launch()` into a graph.
