@@ -43,6 +43,33 @@ struct Task {
     int row, j0, j1;
 };
 
+// Footprint slot table of one piece, lane = slot q: t0 the first voxel whose
+// footprint starts at pixel p0 - 2 + q, t1 the second (if any), each as
+// (wl, wr, w3, LDS offset of the voxel's staged slices).
+struct Slots {
+    float4 t0, t1;
+};
+
+// Value of lane - 1 (lane 0: 0), by a DPP wavefront shift.
+__device__ __forceinline__ int lane_shift_up(int v)
+{
+    constexpr int kWaveShr1 = 0x138;
+    return __builtin_amdgcn_update_dpp(0, v, kWaveShr1, 0xf, 0xf, false);
+}
+
+// Cross-lane scatter of (a, b, c, o): lane d/4 of the result receives the
+// values of the highest-numbered lane whose byte address d points at it;
+// lanes that no lane points at receive zeros (ds_permute_b32 semantics).
+__device__ __forceinline__ float4 permute4(int d, float a, float b, float c, int o)
+{
+    float4 r;
+    r.x = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(d, __builtin_bit_cast(int, a)));
+    r.y = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(d, __builtin_bit_cast(int, b)));
+    r.z = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(d, __builtin_bit_cast(int, c)));
+    r.w = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(d, o));
+    return r;
+}
+
 // Block = NW angles (one per wave) x P detector pixels x 64*HL slices.
 // For every line of the volume along the angles' dominant axis, the union of
 // the NW line segments that touch the block's pixels is staged in LDS once
@@ -82,9 +109,6 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     __shared__ float4 s_xb[STAGE / 4];
     __shared__ AngleK s_k[NW];
     __shared__ int s_mode[NW];
-    // Per wave, double-buffered: footprint slots [rank][first pixel q] =
-    // (wl, wr, w3, LDS offset).
-    __shared__ float4 s_t[NW][2][2][P + 2];  // [wave][table buffer][rank][q]
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -294,12 +318,14 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
             }
         };
 
-        // Footprints of this wave's voxels in piece t, scattered into slot
-        // table tb: slot (rank, q) holds the rank-th voxel whose footprint
-        // starts at pixel p0 - 2 + q (at most two per pixel); empty slots keep
-        // zero weights and a valid dummy offset, so the sweep is straight-line
-        // code.  Returns whether the wave has anything to add from this piece.
-        auto setup = [&](const Task& t, int tb) -> int {
+        // Footprints of this wave's voxels in piece t (lane = voxel), moved
+        // into the slot table st (lane = slot) by cross-lane permutes: slot
+        // (rank, q) holds the rank-th voxel whose footprint starts at pixel
+        // p0 - 2 + q (at most two per pixel) as (wl, wr, w3, LDS offset).
+        // Empty slots hold zero weights and offset 0, so the sweep is
+        // straight-line code.  Returns whether the wave has anything to add
+        // from this piece.
+        auto setup = [&](const Task& t, Slots& st) -> int {
             if (!mine) return 0;
             const int jlo = uni(__builtin_amdgcn_readlane(s_lo, t.row - c0));
             const int jhi = uni(__builtin_amdgcn_readlane(s_hi, t.row - c0));
@@ -312,7 +338,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
             // covers l in [p0-2, p0+P-1], the voxels that touch the block's pixels
             const Taps2 tp = anchor_left(
                 voxel_taps(px_ ? (float)jm - xc : R, px_ ? R : (float)jm - yc, k));
-            const int lprev = __shfl_up(tp.l, 1);  // lane m-1 holds the previous voxel
+            const int lprev = lane_shift_up(tp.l);  // lane m-1 holds the previous voxel
             const bool valid = lane < n;
             const int l = valid ? tp.l : 0x3fffffff;
             const int mm0 = uni((int)__popcll(__ballot(valid && l < pl - 1)));
@@ -322,54 +348,51 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
             mA = uni(max(mA, mm0));
             mB = uni(min(mB, mend - 1));
             if (mA > mB) return 0;
-            float4* tab = &s_t[wave][tb][0][0];
-            for (int i = lane; i < 2 * (P + 2); i += 64) tab[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             const int jb0 = fwd ? (jlo - t.j0) : (jhi - t.j0);
             const int dj = fwd ? 1 : -1;
-            if (lane >= mA && lane <= mB) {
-                const int q = l - (pl - 1);
-                const int rank = (lane > mA && l == lprev) ? 1 : 0;
-                const int off = (jb0 + dj * lane) * VOX;
-                tab[rank * (P + 2) + q] = make_float4(tp.wl, tp.wr, tp.w3, __builtin_bit_cast(float, off));
-            }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // Destination lane of each voxel in the rank-0 and rank-1 tables;
+            // voxels outside [mA, mB] go to lane 63, which is never a slot.
+            const bool in = lane >= mA && lane <= mB;
+            const bool second = lane > mA && l == lprev;
+            const int q = l - (pl - 1);
+            const int d0 = 4 * ((in && !second) ? q : 63);
+            const int d1 = 4 * ((in && second) ? q : 63);
+            const int off = (jb0 + dj * lane) * VOX;
+            st.t0 = permute4(d0, tp.wl, tp.wr, tp.w3, off);
+            st.t1 = permute4(d1, tp.wl, tp.wr, tp.w3, off);
             return 1;
         };
 
         // Pipeline over pieces: while one piece is swept from one buffer, the
-        // next streams into the other buffer and its slot tables are built.
-        __syncthreads();  // previous pass's readers of s_xa / s_xb / s_t are done
+        // next streams into the other buffer and its slot table is built.
+        __syncthreads();  // previous pass's readers of s_xa / s_xb are done
         Task cur = first_task(0);
         int flag = 0;
+        Slots sl;
         if (cur.row >= 0) {
             fetch(cur, s_xa);
-            flag = setup(cur, 0);
+            flag = setup(cur, sl);
         }
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         int rows_done = 0;
-        // One piece: sweep `cur` from xcur with slot table tb; stage the next
-        // piece into xnxt / table tb^1.  Inlined at two call sites so that
-        // both buffers are compile-time known.
-        auto step = [&](const float4* xcur, float4* xnxt, const int tb) __attribute__((always_inline)) {
-            // The table stores come before the DMA is issued, so nothing
-            // between the DMA and the barrier has to wait for it.
+        // One piece: sweep `cur` from xcur with slot table sl; stage the next
+        // piece into xnxt and build its table.  Inlined at two call sites so
+        // that both buffers are compile-time known.
+        auto step = [&](const float4* xcur, float4* xnxt) __attribute__((always_inline)) {
             const Task nxt = next_task(cur);
             int flag_n = 0;
+            Slots sl_n;
             if (nxt.row >= 0) {
-                flag_n = setup(nxt, tb ^ 1);
+                flag_n = setup(nxt, sl_n);
                 fetch(nxt, xnxt);
             }
             if (flag) {
-                const float4* tab = &s_t[wave][tb][0][0];
-                const int ql = min(lane, P + 1);
-                const float4 t0 = tab[ql];
-                const float4 t1 = tab[(P + 2) + ql];
+                constexpr int kMaxOff = (UMAX - 1) * VOX;
+                float4 t0 = sl.t0;
+                const float4 t1 = sl.t1;
+                // staged-buffer offsets of the rank-0 voxels, kept in range
+                t0.w = __builtin_bit_cast(float, min(max(__builtin_bit_cast(int, t0.w), 0), kMaxOff));
                 // bit q: slot q holds a second voxel (about 1 in 10 at 512^3 x 192)
                 const uint64_t has2 = __ballot(lane < P + 2 && (t1.x != 0.f || t1.y != 0.f || t1.z != 0.f));
                 // bit q: the rank-0 / rank-1 voxel of slot q covers three pixels
@@ -399,7 +422,6 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                 // Software-pipelined sweep: the two slot rows of pixel q+1 are
                 // fetched (unconditionally, clamped to the staged buffer)
                 // before the FMAs of pixel q.
-                constexpr int kMaxOff = (UMAX - 1) * VOX;
                 auto slot_offs = [&](int q, int& o0, int& o1) {
                     o0 = __builtin_bit_cast(int, rdlane(t0.w, q));
                     o1 = min(max(o0 + dvox, 0), kMaxOff);
@@ -451,6 +473,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
             __syncthreads();
             cur = nxt;
             flag = flag_n;
+            sl = sl_n;
         };
         while (cur.row >= 0) {
             // Fold the partial sums when a new group of kRowsPerFold lines
@@ -461,9 +484,9 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                 fold();
                 rows_done = cur.row;
             }
-            step(s_xa, s_xb, 0);
+            step(s_xa, s_xb);
             if (cur.row < 0) break;
-            step(s_xb, s_xa, 1);
+            step(s_xb, s_xa);
         }
         fold();
     }
