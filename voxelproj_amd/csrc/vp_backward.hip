@@ -215,34 +215,46 @@ __device__ __forceinline__ void backproject_block(const OpArgs& g, const int log
             int off = p0 - s_rlo[tb][ae];
             off = min(max(off, 0), WIN - 3);
             const int foff = (al * WIN + off) * VOX;  // LDS float offset of pixel r-1
+            // Taps (pixels r-1, r, r+1; HL slices each) of item L of this
+            // lane sweep, read from the staged windows.  Software-pipelined:
+            // the taps of item L+1 are loaded before the FMAs of item L.
+            struct Y3 {
+                float t[3][HL];
+            };
+            auto load_taps = [&](int L) {
+                const float* pr = sy + rdlane(foff, L);
+                Y3 o;
+#pragma unroll
+                for (int k3 = 0; k3 < 3; ++k3) {
+                    if constexpr (HL == 2) {
+                        const float2 v2 = *reinterpret_cast<const float2*>(pr + k3 * VOX);
+                        o.t[k3][0] = v2.x;
+                        o.t[k3][1] = v2.y;
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < HL; ++j) o.t[k3][j] = pr[k3 * VOX + j];
+                    }
+                }
+                return o;
+            };
+            Y3 nxt = load_taps(0);
 #pragma unroll
             for (int s2 = 0; s2 < APL; ++s2) {
 #pragma unroll
                 for (int v = 0; v < VPW; ++v) {
                     const int L = s2 * VPW + v;
-                    const float* pr = sy + rdlane(foff, L);
+                    const Y3 cur = nxt;
+                    if (L + 1 < APL * VPW) nxt = load_taps(L + 1);
                     const float a0 = rdlane(wm, L);
                     const float a1 = rdlane(w0, L);
                     const float a2 = rdlane(wp, L);
-                    if constexpr (HL == 2) {
-                        const float2 y0 = *reinterpret_cast<const float2*>(pr);
-                        const float2 y1 = *reinterpret_cast<const float2*>(pr + VOX);
-                        const float2 y2 = *reinterpret_cast<const float2*>(pr + 2 * VOX);
-                        part[v][0] = fmaf(a0, y0.x, part[v][0]);
-                        part[v][1] = fmaf(a0, y0.y, part[v][1]);
-                        part[v][0] = fmaf(a1, y1.x, part[v][0]);
-                        part[v][1] = fmaf(a1, y1.y, part[v][1]);
-                        part[v][0] = fmaf(a2, y2.x, part[v][0]);
-                        part[v][1] = fmaf(a2, y2.y, part[v][1]);
-                    } else {
 #pragma unroll
-                        for (int j = 0; j < HL; ++j) {
-                            float s = part[v][j];
-                            s = fmaf(a0, pr[j], s);
-                            s = fmaf(a1, pr[VOX + j], s);
-                            s = fmaf(a2, pr[2 * VOX + j], s);
-                            part[v][j] = s;
-                        }
+                    for (int j = 0; j < HL; ++j) {
+                        float s = part[v][j];
+                        s = fmaf(a0, cur.t[0][j], s);
+                        s = fmaf(a1, cur.t[1][j], s);
+                        s = fmaf(a2, cur.t[2][j], s);
+                        part[v][j] = s;
                     }
                 }
             }
