@@ -375,7 +375,7 @@ __global__ __launch_bounds__(64 * NW) void backproject_batched_kernel(
 }
 
 // Tile configurations.  2 slices per lane: 16 x 24 voxels and 12 waves, so
-// 3 waves share a SIMD (166 VGPRs) and the staged window serves 384 voxels
+// 3 waves share a SIMD (168 VGPRs) and the staged window serves 384 voxels
 // (measured 5 % faster than 16 x 16 with 8 waves, bitwise-equal results);
 // 1 slice per lane keeps 16 x 16 with 8 waves (bigger tiles measured slower).
 template <int HL>
