@@ -133,7 +133,10 @@ VP_API int vp_forward_batched(int variant, const vp_batch_item *items_host,
 VP_API int vp_backward_batched(int variant, const vp_batch_item *items_host,
                                const vp_batch_item *items_dev, int n, int grid, void *stream);
 
-/* Kernel variant used by vp_forward (op 0) / vp_backward (op 1) for H slices. */
+/* Default kernel variant (slices per lane) of op 0 (A) / 1 (AT) for H slices,
+ * the grouping key of batched launches.  vp_forward / vp_backward use it too,
+ * except that 2 drops to 1 when the launch would have fewer workgroups than
+ * the device has compute units (the two variants give identical results). */
 VP_API int vp_default_variant(int op, int64_t H);
 
 /* Number of work items (workgroups) one instance contributes to a batched

@@ -124,9 +124,11 @@ def test_plan_validation():
 
 
 def test_grouped_plan_equals_plain_operator_bitwise():
-    """Grouping by variant gives every instance the single-instance operator's
-    variant, so the batched results are the plain results bit for bit; the
-    grouped A/AT pair also replays exactly from a graph."""
+    """Grouped launches give the plain results bit for bit: grouping uses the
+    plain operator's variant by H, and where a single launch of a small
+    volume drops from 2 slices per lane to 1, the two variants add the same
+    terms in the same order.  The grouped A/AT pair also replays exactly from
+    a graph."""
     xs, _, angs = make_instances(6)
     ys = [torch.empty((a.numel(), s[4], s[2]), device=DEV) for a, s in zip(angs, SHAPES)]
     xb = [torch.empty_like(x) for x in xs]

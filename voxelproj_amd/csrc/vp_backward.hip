@@ -23,6 +23,7 @@
 #include "vp_launch.hpp"
 
 #include <algorithm>
+#include <atomic>
 
 namespace vp {
 namespace {
@@ -422,6 +423,29 @@ int default_variant(int op, int64_t H)
     return H > 64 ? 2 : 1;
 }
 
+int device_cus()
+{
+    constexpr int kMaxDevices = 64;
+    static std::atomic<int> cached[kMaxDevices];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return 256;
+    int n = cached[dev].load(std::memory_order_relaxed);
+    if (n <= 0) {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+            n = 256;
+        cached[dev].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
+int single_variant(int op, int A, int64_t SY, int64_t SX, int64_t H, int64_t PX)
+{
+    const int v = default_variant(op, H);
+    if (v != 2) return v;
+    const int64_t nb = op == 0 ? forward_blocks(2, A, H, PX) : backward_blocks(2, SY, SX, H);
+    return nb < device_cus() ? 1 : 2;
+}
+
 hipError_t launch_backward_z0(const float* y, float* x, const float* angles, const Geom& g,
                               int accumulate, hipStream_t s)
 {
@@ -438,7 +462,8 @@ hipError_t launch_backward_z0(const float* y, float* x, const float* angles, con
     a.scale = ldexp(1.0, g.bits);
     a.inv_scale = ldexp(1.0, -g.bits);
     a.du = g.du;
-    return default_variant(1, g.H) == 1 ? backward_single<1>(a, s) : backward_single<2>(a, s);
+    return single_variant(1, g.A, g.SY, g.SX, g.H, g.PX) == 1 ? backward_single<1>(a, s)
+                                                              : backward_single<2>(a, s);
 }
 
 hipError_t launch_backward_batched(int variant, const BatchItem* items_host,

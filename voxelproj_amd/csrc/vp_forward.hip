@@ -628,7 +628,7 @@ hipError_t launch_forward_z0(const float* x, float* y, const float* angles, cons
     a.scale = ldexp(1.0, g.bits);
     a.inv_scale = ldexp(1.0, -g.bits);
     a.du = g.du;
-    switch (default_variant(0, g.H)) {
+    switch (single_variant(0, g.A, g.SY, g.SX, g.H, g.PX)) {
         case 1: return forward_single<1>(a, s);
         case 2: return forward_single<2>(a, s);
         default: return forward_single<4>(a, s);

@@ -37,6 +37,11 @@ struct BatchItem {
 constexpr int kForwardVariants[] = {1, 2, 4};
 constexpr int kBackwardVariants[] = {1, 2};
 int default_variant(int op, int64_t H);  // op 0 = A, 1 = AT
+// Variant of a single (non-batched) launch: the H-based default, except that
+// 2 slices per lane drops to 1 when its grid has fewer workgroups than the
+// device has CUs (small volumes: twice the workgroups fill more of the GPU).
+int single_variant(int op, int A, int64_t SY, int64_t SX, int64_t H, int64_t PX);
+int device_cus();  // compute units of the current device (cached per device)
 // Work items (blocks) of one instance under a variant; -1 for an unknown variant.
 int64_t forward_blocks(int variant, int A, int64_t H, int64_t PX);
 int64_t backward_blocks(int variant, int64_t SY, int64_t SX, int64_t H);
