@@ -108,6 +108,43 @@ def cpu_baseline(N, A, H_sample, budget_s=10.0):
     }
 
 
+# The reference's published single-op times (README.md:58-59, RTX 3060,
+# z_order 0, 192 angles, ms) for the cubic sizes it reports.
+REFERENCE_MS = {128: (8.5, 7.7), 256: (59.8, 50.3), 512: (516.0, 397.0)}
+
+
+def bench_configs(dev, stream, n_t=20):
+    """A and AT (z_order 0) alone at the other workload sizes: 128^3 and 256^3
+    x 192 angles (linspace(0, pi)) and 1024^2 x 512 slices x 1800 angles
+    (linspace(0, pi) without the endpoint), PX = max(SY, SX); median of n_t
+    calls (HIP events)."""
+    from voxelproj_amd import _CU
+    res = {}
+    for SY, SX, H, A, endpoint in ((128, 128, 128, 192, True), (256, 256, 256, 192, True),
+                                   (1024, 1024, 512, 1800, False)):
+        PX = max(SY, SX)
+        ang = torch.tensor(np.linspace(0, np.pi, A, endpoint=endpoint).astype(np.float32), device=dev)
+        g = torch.Generator(device=dev).manual_seed(1)
+        x = torch.randn((SY, SX, H), device=dev, generator=g)
+        yb = torch.randn((A, PX, H), device=dev, generator=g)
+        y = torch.zeros((A, PX, H), device=dev)
+        xb = torch.zeros((SY, SX, H), device=dev)
+        n = n_t if SY * SX * H * A < 1e11 else 5
+        ms_a = time_op(_CU.forward_z0, (x, y, ang), n, stream)[0]
+        ms_at = time_op(_CU.backward_z0, (yb, xb, ang), n, stream)[0]
+        va = float(SY) * SX * H * A
+        key = f"{SY}x{SX}x{H}x{A}"
+        res[key] = {"A_ms": round(ms_a, 4), "AT_ms": round(ms_at, 4),
+                    "A_gvox_ang_s": round(va / ms_a / 1e6, 1),
+                    "AT_gvox_ang_s": round(va / ms_at / 1e6, 1), "timed_calls": n}
+        if SY == SX == H and A == 192 and SY in REFERENCE_MS:
+            ra, rat = REFERENCE_MS[SY]
+            res[key]["reference_rtx3060_ms"] = [ra, rat]
+        del x, yb, y, xb
+        torch.cuda.synchronize()
+    return res
+
+
 def bench_batched(dev, n=48, seed=0, reps=3):
     """Batched form (uneven volumes): plain one-instance-at-a-time A then AT vs
     the autotuned persistent batched A then AT replayed from a HIP graph."""
@@ -195,6 +232,7 @@ def main():
     ap.add_argument("--no-z2", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the projector path is not sharded across GPUs "
@@ -320,6 +358,8 @@ def main():
         "roofline": roofline,
         "per_op": per_op,
     }
+    if not args.no_configs:
+        out["other_sizes"] = bench_configs(dev, stream)
     if not args.no_batched:
         out["batched"] = bench_batched(dev)
     if not args.no_serving:
