@@ -454,8 +454,19 @@ int single_variant(int op, int A, int64_t SY, int64_t SX, int64_t H, int64_t PX)
 {
     const int v = default_variant(op, H);
     if (v != 2) return v;
-    const int64_t nb = op == 0 ? forward_blocks(2, A, H, PX) : backward_blocks(2, SY, SX, H);
-    return nb < device_cus() ? 1 : 2;
+    const int cus = device_cus();
+    if (op == 0) return forward_blocks(2, A, H, PX) < cus ? 1 : 2;
+    // AT: estimated time = rounds of resident workgroups x time per round.
+    // 2 slices per lane: one 16 x 24 workgroup per CU (LDS); 1 slice per
+    // lane: two 16 x 16 workgroups per CU, each about 13 % less efficient per
+    // voxel-slice (measured at 512^3 x 192).  Picks 1 at 128^3 - 256^3 and 2
+    // from 384^3 up, matching measurements.
+    const double w2 = (double)BwdCfg<2>::TX * BwdCfg<2>::TY * 128;
+    const double w1 = (double)BwdCfg<1>::TX * BwdCfg<1>::TY * 64 / 0.87;
+    const int64_t n2 = backward_blocks(2, SY, SX, H), n1 = backward_blocks(1, SY, SX, H);
+    const double t2 = (double)((n2 + cus - 1) / cus) * w2;
+    const double t1 = (double)((n1 + 2 * cus - 1) / (2 * cus)) * 2.0 * w1;
+    return t1 < t2 ? 1 : 2;
 }
 
 hipError_t launch_backward_z0(const float* y, float* x, const float* angles, const Geom& g,

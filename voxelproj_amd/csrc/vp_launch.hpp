@@ -38,8 +38,9 @@ constexpr int kForwardVariants[] = {1, 2, 4};
 constexpr int kBackwardVariants[] = {1, 2};
 int default_variant(int op, int64_t H);  // op 0 = A, 1 = AT
 // Variant of a single (non-batched) launch: the H-based default, except that
-// 2 slices per lane drops to 1 when its grid has fewer workgroups than the
-// device has CUs (small volumes: twice the workgroups fill more of the GPU).
+// 2 slices per lane drops to 1 for A when its grid has fewer workgroups than
+// the device has CUs, and for AT when an estimate of the rounds of resident
+// workgroups favours it (small and mid-size volumes).
 int single_variant(int op, int A, int64_t SY, int64_t SX, int64_t H, int64_t PX);
 int device_cus();  // compute units of the current device (cached per device)
 // Work items (blocks) of one instance under a variant; -1 for an unknown variant.
