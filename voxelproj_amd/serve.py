@@ -15,9 +15,12 @@ by one step:
 
 Requests that arrive while a tick is on the GPU join the next tick, so work
 of different requests overlaps in the same launches ("continuous batching").
-Every instance runs with the kernel variant chosen from its own shape (one
-launch per variant group), so a request's result does not depend on what
-else is in flight: alone and in the mix it is bitwise identical.
+An instance whose own launch fills the device runs as a plain single launch;
+the others run with the kernel variant chosen from their own shape (one
+batched launch per variant group).  The choice depends only on the
+instance's shape, and both paths add the same terms in the same order, so a
+request's result does not depend on what else is in flight: alone and in
+the mix it is bitwise identical.
 
 Latency = completion - arrival; throughput = voxel-angles / makespan, where
 an A or AT of an (SY, SX, H) volume at n_angles counts SY*SX*H*n_angles and a
@@ -104,6 +107,7 @@ class Server:
         self.mem_budget = int(mem_budget)
         self.max_active = max_active
         self.grid = default_grid(self.device) if grid is None else grid
+        self.cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         self.active = []
         self.in_use = 0
         self.ticks = 0
@@ -137,11 +141,33 @@ class Server:
         self.active.append(req)
 
     # ------------------------------------------------------------ one tick
+    def _fills_device(self, op, inp, out, angles, H):
+        """Whether the instance's own launch has at least one workgroup per CU:
+        then it runs as a plain single launch (batching adds nothing to it)."""
+        if op == "A":
+            (SY, SX, _), PX = inp.shape, out.shape[1]
+        else:
+            (SY, SX, _), PX = out.shape, inp.shape[1]
+        code = 0 if op == "A" else 1
+        nb = _lib.load().vp_batch_blocks(code, _variant(op, H), angles.numel(), SY, SX, H, PX)
+        return nb >= self.cus
+
     def _launch(self, op, jobs):
-        """jobs: list of (in, out, angles, H); one batched launch per variant."""
+        """jobs: list of (in, out, angles, H).  Instances that fill the device
+        on their own run as plain single launches; the rest go into one
+        batched launch per variant.  Both paths add the same terms in the same
+        order, so a result does not depend on which path or batch it took."""
         groups = {}
         for j in jobs:
-            groups.setdefault(_variant(op, j[3]), []).append(j)
+            if self._fills_device(op, *j):
+                inp, out, angles, _H = j
+                out.zero_()
+                if op == "A":
+                    ops.project(inp, angles, out.shape[1], out=out)
+                else:
+                    ops.backproject(inp, angles, out.shape[0], out.shape[1], out=out)
+            else:
+                groups.setdefault(_variant(op, j[3]), []).append(j)
         for v, js in sorted(groups.items()):
             BatchPlan(op, [j[0] for j in js], [j[1] for j in js], [j[2] for j in js],
                       accumulate=False, variant=v, grid=self.grid).launch()
