@@ -1,0 +1,71 @@
+"""Launch configurations of the single-instance operators.
+
+A single launch picks its kernel configuration from the volume and the GPU:
+1 or 2 slices per lane, and for volumes whose grid would leave CUs idle the
+small backprojection tiles (16 x 8) and, up to 128 voxels per side, the
+narrow forward strips (16 pixels).  None of these choices changes the order
+in which an output adds its terms, so every configuration with 1 or 2 slices
+per lane must give the plain operator's result bit for bit.  The batched
+launches with an explicit variant use the default configurations (32-pixel
+strips; 16 x 16 and 16 x 24 tiles), so comparing them with the single launch
+checks exactly that.  (4 slices per lane uses 28-pixel strips and is covered
+by the tolerance tests.)
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as vpo
+from voxelproj_amd import batch, ops
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+# (SY, SX, H, n_angles, PX): tiny (small configurations), small volumes whose
+# default grid has fewer workgroups than CUs, and mid-size volumes on the
+# default configurations; ragged H and truncated detectors included.
+SHAPES = [(24, 20, 9, 30, 32), (96, 96, 64, 120, 96), (128, 128, 128, 192, 128),
+          (72, 200, 37, 77, 150), (256, 256, 40, 360, 256), (160, 160, 100, 192, 120)]
+
+
+def _inputs(shape, seed):
+    SY, SX, H, A, PX = shape
+    g = torch.Generator(device=DEV).manual_seed(seed)
+    x = torch.randn((SY, SX, H), device=DEV, generator=g)
+    y = torch.randn((A, PX, H), device=DEV, generator=g)
+    rng = np.random.default_rng(seed)
+    ang = torch.tensor(np.sort(rng.uniform(0, np.pi, A)).astype(np.float32), device=DEV)
+    return x, y, ang
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_single_launch_equals_every_1_2_slice_configuration(shape):
+    SY, SX, H, A, PX = shape
+    x, y, ang = _inputs(shape, 17)
+    print(f"shape {shape}", flush=True)
+    ya = ops.project(x, ang, PX)
+    xb = ops.backproject(y, ang, SY, SX)
+    for v in (1, 2):
+        yv = torch.empty_like(ya)
+        batch.BatchPlan("A", [x], [yv], [ang], accumulate=False, variant=v).launch()
+        assert torch.equal(yv, ya), ("A", v)
+        xv = torch.empty_like(xb)
+        batch.BatchPlan("AT", [y], [xv], [ang], accumulate=False, variant=v).launch()
+        assert torch.equal(xv, xb), ("AT", v)
+
+
+@pytest.mark.parametrize("shape", SHAPES[:3])
+def test_small_configurations_match_oracle(shape):
+    SY, SX, H, A, PX = shape
+    x, y, ang = _inputs(shape, 5)
+    hs = slice(0, min(H, 6))  # slices are independent: check a few against FP64
+    xs = x[:, :, hs].contiguous()
+    ys = y[:, :, hs].contiguous()
+    a = ang.cpu().numpy()
+    got_y = ops.project(x, ang, PX)[:, :, hs].cpu().numpy()
+    got_x = ops.backproject(y, ang, SY, SX)[:, :, hs].cpu().numpy()
+    ref_y = vpo.forward_f64(xs.cpu().numpy(), a, PX, 0)
+    ref_x = vpo.backward_f64(ys.cpu().numpy(), a, SY, SX, 0)
+    for got, ref in ((got_y, ref_y), (got_x, ref_x)):
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-6
+        assert np.abs(got - ref).max() / np.abs(ref).max() <= 1e-5

@@ -579,10 +579,14 @@ template <> struct FwdCfg<1> { static constexpr int P = 32, NW = 8, UMAX = 128; 
 template <> struct FwdCfg<2> { static constexpr int P = 32, NW = 8, UMAX = 128; };
 template <> struct FwdCfg<4> { static constexpr int P = 28, NW = 8, UMAX = 72; };
 
-// Narrower strips for 1 slice per lane, used by single launches when the
-// default grid would have fewer workgroups than the device has CUs: twice the
-// workgroups, each with half the pixels (measured bitwise equal to 32-pixel
-// strips; 128^3 x 192: 0.46 -> 0.38 ms).
+// Narrower strips for 1 slice per lane, used by single launches of volumes
+// of at most UMAX = 128 voxels per side when the default grid would have fewer
+// workgroups than the device has CUs: twice the workgroups, each with half the
+// pixels (128^3 x 192: 0.46 -> 0.38 ms).  Within that size no line is split
+// into pieces and all lines fall in one fold group, so every output adds the
+// same terms in the same order as with 32-pixel strips: results are equal bit
+// for bit.  (For larger volumes the strip width changes where a line's staged
+// union is split and which pieces pair up around a fold, hence the last bits.)
 struct FwdSmall {
     static constexpr int P = 16, NW = 8, UMAX = 128;
 };
@@ -637,7 +641,8 @@ hipError_t launch_forward_z0(const float* x, float* y, const float* angles, cons
     a.du = g.du;
     switch (single_variant(0, g.A, g.SY, g.SX, g.H, g.PX)) {
         case 1:
-            if (forward_blocks(1, g.A, g.H, g.PX) < device_cus()) return forward_single<1, FwdSmall>(a, s);
+            if (std::max(g.SY, g.SX) <= FwdSmall::UMAX && forward_blocks(1, g.A, g.H, g.PX) < device_cus())
+                return forward_single<1, FwdSmall>(a, s);
             return forward_single<1>(a, s);
         case 2: return forward_single<2>(a, s);
         default: return forward_single<4>(a, s);
