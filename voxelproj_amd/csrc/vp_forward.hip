@@ -20,6 +20,7 @@
 #include "vp_launch.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace vp {
 namespace {
@@ -394,10 +395,13 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                 // staged-buffer offsets of the rank-0 voxels, kept in range
                 t0.w = __builtin_bit_cast(float, min(max(__builtin_bit_cast(int, t0.w), 0), kMaxOff));
                 // bit q: slot q holds a second voxel (about 1 in 10 at 512^3 x 192)
-                const uint64_t has2 = __ballot(lane < P + 2 && (t1.x != 0.f || t1.y != 0.f || t1.z != 0.f));
+                // (32-bit masks when the slots fit: one scalar bit test each)
+                using Mask = std::conditional_t<(P + 2 <= 32), uint32_t, uint64_t>;
+                const Mask has2 = (Mask)__ballot(lane < P + 2 && (t1.x != 0.f || t1.y != 0.f || t1.z != 0.f));
                 // bit q: the rank-0 / rank-1 voxel of slot q covers three pixels
-                const uint64_t has3a = __ballot(lane < P + 2 && t0.z != 0.f);
-                const uint64_t has3b = __ballot(lane < P + 2 && t1.z != 0.f);
+                const Mask has3a = (Mask)__ballot(lane < P + 2 && t0.z != 0.f);
+                const Mask has3b = (Mask)__ballot(lane < P + 2 && t1.z != 0.f);
+                const Mask extra = has3a | has2;  // has3b implies has2
                 const int dvox = fwd ? VOX : -VOX;
                 const float* sx = reinterpret_cast<const float*>(xcur) + lane * HL;
                 auto ld = [&](int off) {
@@ -430,41 +434,48 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                 slot_offs(0, o0, o1);
                 auto xa_n = ld(o0);
                 auto xb_n = ld(o1);
+                // the rank-0 weights of pixel q+1 are read out one pixel ahead too
+                float wl_n = rdlane(t0.x, 0), wr_n = rdlane(t0.y, 0);
 #pragma unroll
                 for (int q = 0; q < P + 2; ++q) {
                     const auto xa = xa_n;
                     const auto xb = xb_n;
+                    const float wl = wl_n, wr = wr_n;
                     if (q + 1 < P + 2) {
                         slot_offs(q + 1, o0, o1);
                         xa_n = ld(o0);
                         xb_n = ld(o1);
+                        wl_n = rdlane(t0.x, q + 1);
+                        wr_n = rdlane(t0.y, q + 1);
                     }
                     {
-                        const float wl = rdlane(t0.x, q);
-                        const float wr = rdlane(t0.y, q);
 #pragma unroll
                         for (int j = 0; j < HL; ++j) {
                             if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wl, xa.a[j], part[q - 2][j]);
                             if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(wr, xa.a[j], part[q - 1][j]);
                         }
-                        if (q < P && ((has3a >> q) & 1ull)) {
+                    }
+                    // one test for the common case of a slot with neither a
+                    // three-pixel rank-0 footprint nor a second voxel
+                    if ((extra >> q) & 1u) {
+                        if (q < P && ((has3a >> q) & 1u)) {
                             const float w3 = rdlane(t0.z, q);
 #pragma unroll
                             for (int j = 0; j < HL; ++j) part[q][j] = fmaf(w3, xa.a[j], part[q][j]);
                         }
-                    }
-                    if ((has2 >> q) & 1ull) {
-                        const float wl = rdlane(t1.x, q);
-                        const float wr = rdlane(t1.y, q);
+                        if ((has2 >> q) & 1u) {
+                            const float wl = rdlane(t1.x, q);
+                            const float wr = rdlane(t1.y, q);
 #pragma unroll
-                        for (int j = 0; j < HL; ++j) {
-                            if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wl, xb.a[j], part[q - 2][j]);
-                            if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(wr, xb.a[j], part[q - 1][j]);
-                        }
-                        if (q < P && ((has3b >> q) & 1ull)) {
-                            const float w3 = rdlane(t1.z, q);
+                            for (int j = 0; j < HL; ++j) {
+                                if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wl, xb.a[j], part[q - 2][j]);
+                                if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(wr, xb.a[j], part[q - 1][j]);
+                            }
+                            if (q < P && ((has3b >> q) & 1u)) {
+                                const float w3 = rdlane(t1.z, q);
 #pragma unroll
-                            for (int j = 0; j < HL; ++j) part[q][j] = fmaf(w3, xb.a[j], part[q][j]);
+                                for (int j = 0; j < HL; ++j) part[q][j] = fmaf(w3, xb.a[j], part[q][j]);
+                            }
                         }
                     }
                 }
