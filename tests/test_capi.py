@@ -104,3 +104,22 @@ def test_vshift_validates():
     assert lib.vp_vshift(0, 0, p, None, 3, 5, 4, 0, 4, 0, p, 0, None) == -1
     assert lib.vp_vshift(0, 0, p, p, 3, -5, 4, 0, 4, 0, p, 0, None) == -1
     assert lib.vp_vshift(0, 0, None, None, 3, 5, 4, 0, 0, 0, None, 0, None) == 0  # empty output
+
+
+class _BatchItem(ctypes.Structure):
+    _fields_ = [("inp", ctypes.c_void_p), ("out", ctypes.c_void_p), ("angles", ctypes.c_void_p),
+                ("n_angles", ctypes.c_int32), ("SY", ctypes.c_int32), ("SX", ctypes.c_int32),
+                ("H", ctypes.c_int32), ("PX", ctypes.c_int32), ("accumulate", ctypes.c_int32)]
+
+
+def test_batched_forward_offset_span_limit():
+    """Forward variants 1 and 2 stage several voxels per DMA instruction through
+    one 32-bit buffer offset; items whose SX * H exceeds that are rejected
+    (variant 4, one voxel per instruction, takes them)."""
+    lib = voxelproj_amd.load()
+    big = (_BatchItem * 1)(_BatchItem(1, 1, 1, 1, 1, 32768, 16384, 1, 0))
+    for v in (1, 2):
+        assert lib.vp_forward_batched(v, big, big, 1, 0, None) == -2
+        assert b"too large for variant" in lib.vp_last_error()
+    bad = (_BatchItem * 1)(_BatchItem(1, 1, 1, 1, 1, 4, 2, 1, 5))
+    assert lib.vp_forward_batched(1, bad, bad, 1, 0, None) == -1  # accumulate not 0 / 1

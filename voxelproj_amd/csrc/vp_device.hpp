@@ -152,4 +152,25 @@ __host__ __device__ inline int hi_bits(int64_t SY, int64_t SX, int64_t PX, doubl
     return 23 - e;
 }
 
+// Global -> LDS DMA of BYTES (4 or 16) per lane: lane data from base + voff
+// (bytes) lands at the lane's slot of the instruction's LDS destination.
+// `base` must be wave-uniform; reads at voff >= range return 0.  This is the
+// buffer (MUBUF) form of the transfer: the compiler tracks it with vmcnt
+// only, so LDS reads issued while it is in flight keep their pipelined
+// lgkmcnt waits.  With the FLAT form (global_load_lds) in flight it treats
+// the LDS counter as unordered and makes every LDS wait lgkmcnt(0).
+template <int BYTES>
+__device__ __forceinline__ void dma_to_lds(const void* base, uint32_t range, uint32_t voff,
+                                           __attribute__((address_space(3))) void* lds)
+{
+    constexpr int kRsrcWord3 = 0x00020000;  // raw 32-bit buffer, gfx9 family
+    const __amdgpu_buffer_rsrc_t r =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)range, kRsrcWord3);
+    static_assert(BYTES == 4 || BYTES == 16, "DMA moves 4 or 16 bytes per lane");
+    if constexpr (BYTES == 16)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(r, lds, 16, voff, 0, 0, 0);
+    else
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(r, lds, 4, voff, 0, 0, 0);
+}
+
 }  // namespace vp

@@ -289,6 +289,10 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
         auto fetch = [&](const Task& t, float4* dst) __attribute__((always_inline)) {
             const int nv = t.j1 - t.j0 + 1;
             const float* base = x + (int64_t)t.row * st_row + (int64_t)t.j0 * st_along;
+            // Buffer-form DMA: each instruction's base is its first voxel;
+            // lane offsets stay below (RPI-1) * st_along + H floats, under
+            // 2^29 (RPI > 1 only for up to 128 slices in single launches;
+            // batched launches check it).
             if (vec4) {
                 constexpr int RPI = 1024 / (VOX * 4);  // voxels per 16-byte-per-lane DMA
                 constexpr int LPR = 64 / RPI;          // lanes per voxel
@@ -296,11 +300,13 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                 const int sub = lane / LPR;
                 const int hoff = min(hb + (lane % LPR) * 4, H - 4);
                 const int nins = (nv + RPI - 1) / RPI;
+                const uint32_t range = (uint32_t)(((int64_t)(RPI - 1) * st_along + H) * 4);
                 for (int ins = wave; ins < nins; ins += NW) {
-                    const int v = min(ins * RPI + sub, nv - 1);
-                    __builtin_amdgcn_global_load_lds(
-                        (const void*)(base + (int64_t)v * st_along + hoff),
-                        (__attribute__((address_space(3))) void*)&dst[ins * RPI * (VOX / 4)], 16, 0, 0);
+                    const int v0 = ins * RPI;
+                    const int v = min(v0 + sub, nv - 1);
+                    dma_to_lds<16>(base + (int64_t)v0 * st_along, range,
+                                   (uint32_t)(((int64_t)(v - v0) * st_along + hoff) * 4),
+                                   (__attribute__((address_space(3))) void*)&dst[ins * RPI * (VOX / 4)]);
                 }
             } else {
                 for (int v = wave; v < nv; v += NW) {
@@ -310,10 +316,8 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                         // voxel's row, i.e. slices hb + 64c + lane.
                         const int e = 64 * c + lane;           // float index in the row
                         const int hh = min(hb + e, H - 1);
-                        __builtin_amdgcn_global_load_lds(
-                            (const void*)(base + (int64_t)v * st_along + hh),
-                            (__attribute__((address_space(3))) void*)(reinterpret_cast<float*>(&dst[v * (VOX / 4)]) + 64 * c),
-                            4, 0, 0);
+                        dma_to_lds<4>(base + (int64_t)v * st_along, (uint32_t)H * 4, (uint32_t)hh * 4,
+                                      (__attribute__((address_space(3))) void*)(reinterpret_cast<float*>(&dst[v * (VOX / 4)]) + 64 * c));
                     }
                 }
             }
