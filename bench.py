@@ -191,7 +191,9 @@ def bench_batched(dev, n=48, seed=0, reps=3):
 
 def bench_serving(dev, n=40, rate=200.0, seed=0):
     """Serving form: Poisson stream (rate req/s) of 40% A / 40% AT / 20% SIRT-10
-    requests; continuous batching vs one request at a time."""
+    requests; continuous batching vs one request at a time (throughput: back
+    to back from time 0; latency: the same arrivals served first-in
+    first-out without batching)."""
     from voxelproj_amd import serve
 
     reqs = serve.mixed_workload(n, rate=rate, seed=seed, device=dev)
@@ -202,6 +204,7 @@ def bench_serving(dev, n=40, rate=200.0, seed=0):
     server.run(reqs)
     st = serve.latency_stats(reqs)
     _, t_plain = serve.run_plain(reqs)
+    fifo = serve.run_plain_fifo(reqs)
     served = serve.throughput(reqs, server.makespan)
     plain = serve.throughput(reqs, t_plain)
     return {
@@ -214,6 +217,10 @@ def bench_serving(dev, n=40, rate=200.0, seed=0):
         "served_gvox_ang_s": round(served, 1),
         "plain_gvox_ang_s": round(plain, 1),
         "speedup": round(served / plain, 3),
+        # the same arrival stream served one request at a time (no batching)
+        "fifo_plain_p50_ms": round(fifo["p50_ms"], 2),
+        "fifo_plain_p99_ms": round(fifo["p99_ms"], 2),
+        "fifo_plain_makespan_s": round(fifo["makespan_s"], 3),
     }
 
 

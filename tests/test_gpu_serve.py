@@ -65,3 +65,13 @@ def test_latency_and_throughput_reporting():
     st = serve.latency_stats(reqs)
     assert 0 <= st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
     assert serve.throughput(reqs, server.makespan) > 0
+
+
+def test_fifo_plain_baseline():
+    """The unbatched first-in-first-out baseline serves the same arrivals: no
+    request completes before it arrives, and the makespan covers the last
+    arrival and all the work."""
+    reqs = small_mix(8, seed=7)
+    fifo = serve.run_plain_fifo(reqs)
+    assert 0 <= fifo["p50_ms"] <= fifo["p99_ms"]
+    assert fifo["makespan_s"] >= max(r.arrival for r in reqs)

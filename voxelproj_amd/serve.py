@@ -173,20 +173,31 @@ class Server:
                       accumulate=False, variant=v, grid=self.grid).launch()
 
     def tick(self):
-        a_jobs, at_jobs = [], []
+        """Advance every admitted request by one step.  Single-operator
+        requests are launched first (A, then AT), so they complete before the
+        SIRT steps of the same tick; returns [(request, completion event)]."""
+        a_plain, at_plain, a_sirt, at_sirt = [], [], [], []
         for r in self.active:
             st = r.state
             if r.kind == "A":
-                a_jobs.append((r.data, r.result, r.angles, r.H))
+                a_plain.append((r.data, r.result, r.angles, r.H))
             elif r.kind == "AT":
-                at_jobs.append((r.data, r.result, r.angles, r.H))
+                at_plain.append((r.data, r.result, r.angles, r.H))
             elif st["it"] < 0:
-                a_jobs.append((st["ones_v"], st["R"], r.angles, r.H))
-                at_jobs.append((st["ones_s"], st["C"], r.angles, r.H))
+                a_sirt.append((st["ones_v"], st["R"], r.angles, r.H))
+                at_sirt.append((st["ones_s"], st["C"], r.angles, r.H))
             else:
-                a_jobs.append((st["x"], st["ts"], r.angles, r.H))
-        if a_jobs:
-            self._launch("A", a_jobs)
+                a_sirt.append((st["x"], st["ts"], r.angles, r.H))
+        ev_plain = None
+        if a_plain:
+            self._launch("A", a_plain)
+        if at_plain:
+            self._launch("AT", at_plain)
+        if a_plain or at_plain:
+            ev_plain = torch.cuda.Event(enable_timing=True)
+            ev_plain.record()
+        if a_sirt:
+            self._launch("A", a_sirt)
         for r in self.active:
             st = r.state
             if r.kind == "SIRT":
@@ -195,9 +206,9 @@ class Server:
                 else:
                     torch.sub(r.data, st["ts"], out=st["ts"])
                     st["ts"].mul_(st["R"])
-                    at_jobs.append((st["ts"], st["tv"], r.angles, r.H))
-        if at_jobs:
-            self._launch("AT", at_jobs)
+                    at_sirt.append((st["ts"], st["tv"], r.angles, r.H))
+        if at_sirt:
+            self._launch("AT", at_sirt)
         done = []
         for r in self.active:
             st = r.state
@@ -211,8 +222,10 @@ class Server:
                 st["it"] += 1
                 if st["it"] >= r.n_iter:
                     done.append(r)
+        ev_end = torch.cuda.Event(enable_timing=True)
+        ev_end.record()
         self.ticks += 1
-        return done
+        return [(r, ev_plain if r.kind != "SIRT" else ev_end) for r in done]
 
     def _retire(self, r, now):
         r.t_done = now
@@ -230,6 +243,10 @@ class Server:
                 raise RuntimeError(f"request {r.rid} needs {r.bytes_needed()} B, over the budget")
         torch.cuda.synchronize(self.device)
         t0 = time.perf_counter()
+        # completion = when the GPU finished the request's last launch, on the
+        # GPU clock relative to this event (recorded on the idle stream at t0)
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev0.record()
         i = 0
         while i < len(pending) or self.active:
             now = time.perf_counter() - t0
@@ -242,9 +259,8 @@ class Server:
                 continue
             done = self.tick()
             torch.cuda.synchronize(self.device)
-            now = time.perf_counter() - t0
-            for r in done:
-                self._retire(r, now)
+            for r, ev in done:
+                self._retire(r, ev0.elapsed_time(ev) * 1e-3)
         self.makespan = time.perf_counter() - t0
         return requests
 
@@ -257,24 +273,48 @@ def run_alone(req, device="cuda"):
     return clone.result
 
 
+def _run_one_plain(r):
+    if r.kind == "A":
+        return ops.project(r.data, r.angles, r.PX)
+    if r.kind == "AT":
+        return ops.backproject(r.data, r.angles, r.SY, r.SX)
+    solver = recon.SIRT(r.angles, r.SY, r.SX, r.H, r.PX, device=r.data.device)
+    x = torch.zeros((r.SY, r.SX, r.H), device=r.data.device)
+    solver.iterate(x, r.data, r.n_iter)
+    return x
+
+
 def run_plain(requests):
-    """One request at a time through the plain single-instance operators;
-    returns (results, seconds)."""
+    """One request at a time through the plain single-instance operators,
+    back to back from time 0 (arrival times ignored); returns (results,
+    seconds)."""
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    out = []
-    for r in requests:
-        if r.kind == "A":
-            out.append(ops.project(r.data, r.angles, r.PX))
-        elif r.kind == "AT":
-            out.append(ops.backproject(r.data, r.angles, r.SY, r.SX))
-        else:
-            solver = recon.SIRT(r.angles, r.SY, r.SX, r.H, r.PX, device=r.data.device)
-            x = torch.zeros((r.SY, r.SX, r.H), device=r.data.device)
-            solver.iterate(x, r.data, r.n_iter)
-            out.append(x)
+    out = [_run_one_plain(r) for r in requests]
     torch.cuda.synchronize()
     return out, time.perf_counter() - t0
+
+
+def run_plain_fifo(requests):
+    """A server without batching: requests in arrival order, each run alone
+    through the plain operators once it has arrived and the previous one is
+    done.  Returns {"p50_ms", "p99_ms", "makespan_s"} (latency = completion -
+    arrival), the baseline for the continuous-batching latencies."""
+    pending = sorted(requests, key=lambda r: r.arrival)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lat = []
+    for r in pending:
+        now = time.perf_counter() - t0
+        if r.arrival > now:
+            time.sleep(r.arrival - now)
+        _run_one_plain(r)
+        torch.cuda.synchronize()
+        lat.append((time.perf_counter() - t0 - r.arrival) * 1e3)
+    makespan = time.perf_counter() - t0
+    lat = np.array(lat)
+    return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "makespan_s": makespan}
 
 
 def mixed_workload(n, rate=200.0, seed=0, sizes=(128, 256, 512), heights=(64, 128, 256, 512),
