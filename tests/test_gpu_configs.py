@@ -6,9 +6,9 @@ small backprojection tiles (16 x 8) and, up to 128 voxels per side, the
 narrow forward strips (16 pixels).  None of these choices changes the order
 in which an output adds its terms, so every configuration with 1 or 2 slices
 per lane must give the plain operator's result bit for bit.  The batched
-launches with an explicit variant use the default configurations (32-pixel
+launches with an explicit variant use the default configurations (28-pixel
 strips; 16 x 16 and 16 x 24 tiles), so comparing them with the single launch
-checks exactly that.  (4 slices per lane uses 28-pixel strips and is covered
+checks exactly that.  (4 slices per lane uses 22-pixel strips and is covered
 by the tolerance tests.)
 """
 import numpy as np

@@ -590,8 +590,13 @@ __global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict_
 
 template <int HL>
 struct FwdCfg;
-template <> struct FwdCfg<1> { static constexpr int P = 32, NW = 8, UMAX = 128; };
-template <> struct FwdCfg<2> { static constexpr int P = 32, NW = 8, UMAX = 128; };
+// 1 and 2 slices per lane: 12 angles per block (3 waves per SIMD) with
+// 28-pixel strips; against 8 angles with 32-pixel strips, 10-25 % faster on
+// most mid-size volumes (256^2 - 448^2, 64 - 128 slices), never more than
+// 2 % slower in the shapes measured.  Both variants share P, NW and UMAX, so
+// they add every output's terms in the same order (bitwise-equal results).
+template <> struct FwdCfg<1> { static constexpr int P = 28, NW = 12, UMAX = 128; };
+template <> struct FwdCfg<2> { static constexpr int P = 28, NW = 12, UMAX = 128; };
 // 4 slices per lane: 12 angles (waves) per block at 3 waves per SIMD (168
 // VGPRs; the few spills are outside the sweep) and 22-pixel strips.  Against
 // 8 waves with 28-pixel strips: 6 % faster per unit of work (1024^2 x 512 x
@@ -604,8 +609,8 @@ template <> struct FwdCfg<4> { static constexpr int P = 22, NW = 12, UMAX = 72; 
 // workgroups than the device has CUs: twice the workgroups, each with half the
 // pixels (128^3 x 192: 0.46 -> 0.38 ms).  Within that size no line is split
 // into pieces and all lines fall in one fold group, so every output adds the
-// same terms in the same order as with 32-pixel strips: results are equal bit
-// for bit.  (For larger volumes the strip width changes where a line's staged
+// same terms in the same order as with the default strips: results are equal
+// bit for bit.  (For larger volumes the strip width changes where a line's staged
 // union is split and which pieces pair up around a fold, hence the last bits.)
 struct FwdSmall {
     static constexpr int P = 16, NW = 8, UMAX = 128;
