@@ -375,14 +375,15 @@ __global__ __launch_bounds__(64 * NW) void backproject_batched_kernel(
     }
 }
 
-// Tile configurations.  2 slices per lane: 16 x 24 voxels and 12 waves, so
-// 3 waves share a SIMD (168 VGPRs) and the staged window serves 384 voxels
-// (measured 5 % faster than 16 x 16 with 8 waves, bitwise-equal results);
-// 1 slice per lane keeps 16 x 16 with 8 waves (bigger tiles measured slower).
+// Tile configurations.  2 slices per lane: 16 x 16 voxels and 16 waves of 16
+// voxels each, so 4 waves share a SIMD (104 VGPRs, 137 KB LDS); measured 5 %
+// faster than 16 x 24 voxels with 12 waves of 32 (168 VGPRs) at 512^3 x 192
+// and 2 % faster at 1024^2 x 512 x 1800, bitwise-equal results.  1 slice per
+// lane keeps 16 x 16 with 8 waves (bigger tiles measured slower).
 template <int HL>
 struct BwdCfg;
 template <> struct BwdCfg<1> { static constexpr int TX = 16, TY = 16, NW = 8, AC = 4, WIN = 28; };
-template <> struct BwdCfg<2> { static constexpr int TX = 16, TY = 24, NW = 12, AC = 4, WIN = 32; };
+template <> struct BwdCfg<2> { static constexpr int TX = 16, TY = 16, NW = 16, AC = 4, WIN = 28; };
 
 // Smaller tiles for 1 slice per lane, used by single launches when 16 x 16
 // tiles would give fewer workgroups than the device has CUs (the tile shape
@@ -456,14 +457,16 @@ int single_variant(int op, int A, int64_t SY, int64_t SX, int64_t H, int64_t PX)
     if (v != 2) return v;
     const int cus = device_cus();
     if (op == 0) return forward_blocks(2, A, H, PX) < cus ? 1 : 2;
-    // AT: estimated time = rounds of resident workgroups x time per round.
-    // 2 slices per lane: one 16 x 24 workgroup per CU (LDS); 1 slice per
-    // lane: two 16 x 16 workgroups per CU, each about 13 % less efficient per
-    // voxel-slice (measured at 512^3 x 192).  Picks 1 at 128^3 - 256^3 and 2
-    // from 384^3 up, matching measurements.
-    const double w2 = (double)BwdCfg<2>::TX * BwdCfg<2>::TY * 128;
-    const double w1 = (double)BwdCfg<1>::TX * BwdCfg<1>::TY * 64 / 0.87;
+    // AT: with fewer 2-slice workgroups than CUs, 1 slice per lane.  Else
+    // estimated time = rounds of resident workgroups x time per round:
+    // 2 slices per lane, one workgroup per CU (LDS); 1 slice per lane, two
+    // per CU, each about 22 % less efficient per voxel-slice (measured at
+    // 512^3 x 192).  Picks 1 at 128^3 and 192^3 and 2 from 256^3 up, as
+    // measured best (at 320^3 the two are within 2 %).
     const int64_t n2 = backward_blocks(2, SY, SX, H), n1 = backward_blocks(1, SY, SX, H);
+    if (n2 < cus) return 1;
+    const double w2 = (double)BwdCfg<2>::TX * BwdCfg<2>::TY * 128;
+    const double w1 = (double)BwdCfg<1>::TX * BwdCfg<1>::TY * 64 / 0.78;
     const double t2 = (double)((n2 + cus - 1) / cus) * w2;
     const double t1 = (double)((n1 + 2 * cus - 1) / (2 * cus)) * 2.0 * w1;
     return t1 < t2 ? 1 : 2;
