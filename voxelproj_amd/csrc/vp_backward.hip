@@ -386,10 +386,11 @@ template <> struct BwdCfg<1> { static constexpr int TX = 16, TY = 16, NW = 8, AC
 template <> struct BwdCfg<2> { static constexpr int TX = 16, TY = 16, NW = 16, AC = 4, WIN = 28; };
 
 // Smaller tiles for 1 slice per lane, used by single launches when 16 x 16
-// tiles would give fewer workgroups than the device has CUs (the tile shape
-// does not change any voxel's sum; 128^3 x 192: 0.28 -> 0.23 ms).
+// tiles would give fewer workgroups than the device has CUs: 16 x 8 voxels in
+// 8 waves of 16 (the tile shape does not change any voxel's sum; 128^3 x 192:
+// 0.28 ms with 16 x 16 tiles, 0.23 ms with 4 waves of 32, 0.16 ms).
 struct BwdSmall {
-    static constexpr int TX = 16, TY = 8, NW = 4, AC = 4, WIN = 24;
+    static constexpr int TX = 16, TY = 8, NW = 8, AC = 4, WIN = 24;
 };
 
 template <typename C, int HL>
