@@ -601,8 +601,9 @@ template <> struct FwdCfg<2> { static constexpr int P = 28, NW = 12, UMAX = 128;
 // VGPRs; the few spills are outside the sweep) and 22-pixel strips.  Against
 // 8 waves with 28-pixel strips: 6 % faster per unit of work (1024^2 x 512 x
 // 1800), and at 512^3 x 192 the grid is exactly 3 x 256 workgroups instead of
-// 3.6 x 256 (6.2 vs 7.3 ms); 2-6 % slower at 256^3 and 384^3.
-template <> struct FwdCfg<4> { static constexpr int P = 22, NW = 12, UMAX = 72; };
+// 3.6 x 256 (6.2 vs 7.3 ms); 2-6 % slower at 256^3 and 384^3.  Pieces of up
+// to 76 voxels (152 KB of staging) split fewer lines than 72 (1.7 % faster).
+template <> struct FwdCfg<4> { static constexpr int P = 22, NW = 12, UMAX = 76; };
 
 // Narrower strips for 1 slice per lane, used by single launches of volumes
 // of at most UMAX = 128 voxels per side when the default grid would have fewer
