@@ -226,6 +226,30 @@ def test_z_orders_agree_at_512():
     assert d <= MAX_REL
 
 
+@pytest.mark.parametrize("H", [5, 8])
+def test_z2_many_transpose_tiles(H):
+    """z_order 2 goes through layout transposes; with SY * SX = 2048^2 their
+    tile count along one extent is 65536, beyond a 2-D grid's y limit.  H = 8
+    takes the 16-byte path, H = 5 the element path; both equal z_order 0 bit
+    for bit (the transposes are exact)."""
+    SY = SX = 2048
+    A, PX = 3, 24
+    ang = torch.tensor(np.array([0.1, 1.2, 2.9], np.float32), device=DEV)
+    g = torch.Generator(device=DEV).manual_seed(10)
+    x0 = torch.randn((SY, SX, H), device=DEV, generator=g)
+    y0 = torch.randn((A, PX, H), device=DEV, generator=g)
+    ya = torch.zeros((A, PX, H), device=DEV)
+    _CU.forward_z0(x0, ya, ang)
+    y2 = torch.zeros((H, A, PX), device=DEV)
+    _CU.forward_z2(x0.permute(2, 0, 1).contiguous(), y2, ang)
+    assert torch.equal(y2, ya.permute(2, 0, 1))
+    xa = torch.zeros((SY, SX, H), device=DEV)
+    _CU.backward_z0(y0, xa, ang)
+    x2 = torch.ones((H, SY, SX), device=DEV)
+    _CU.backward_z2(y0.permute(2, 0, 1).contiguous(), x2, ang)
+    assert torch.equal(x2, xa.permute(2, 0, 1) + 1.0)
+
+
 def test_1800_angles_1024():
     """1024 x 1024 x 512 volume, 1800 angles in [0, pi): angle chunking past 512."""
     SY = SX = 1024
