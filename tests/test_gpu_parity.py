@@ -123,6 +123,8 @@ CASES = [
     (31, 37, 5, 50, 80, "wide"),            # detector wider than the diagonal, angles outside [0, pi)
     (1, 1, 1, 7, 3, "linspace"),            # single voxel
     (16, 16, 3, 1, 1, "linspace"),          # single pixel, single angle
+    (40, 36, 131, 30, 44, "random"),        # 4 slices per lane, H % 4 != 0 (element DMA), ragged last block
+    (24, 30, 200, 25, 30, "linspace"),      # 4 slices per lane, 16-byte DMA, idle lanes past H
 ]
 
 
