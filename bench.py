@@ -174,11 +174,15 @@ def bench_batched(dev, n=48, seed=0, reps=3):
     tune_b = batch.autotune_grouped(pb, persist=False)
     graph = batch.capture_pair(pa, pb)
     ms_batch = time_op(graph.replay, (), reps, stream, warmup=1)[0]
+    # the same with the variant groups as parallel graph branches
+    graph_c = batch.capture_pair(pa, pb, concurrent=True)
+    ms_conc = time_op(graph_c.replay, (), reps, stream, warmup=1)[0]
     return {
         "workload": f"{n} volumes, SX=SY in {{96..384}}, H in {{32..256}}, "
                     f"n_angles in {{60..1024}}, A then AT (z_order 0)",
         "plain_ms": round(ms_plain, 3),
         "batched_graph_ms": round(ms_batch, 3),
+        "batched_graph_concurrent_groups_ms": round(ms_conc, 3),
         "plain_gvox_ang_s": round(work / ms_plain / 1e6, 1),
         "batched_gvox_ang_s": round(work / ms_batch / 1e6, 1),
         "speedup": round(ms_plain / ms_batch, 3),

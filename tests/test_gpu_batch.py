@@ -152,3 +152,34 @@ def test_grouped_plan_equals_plain_operator_bitwise():
         torch.cuda.synchronize()
         assert all(torch.equal(u, v) for u, v in zip(ys, ref_y)), rep
         assert all(torch.equal(u, v) for u, v in zip(xb, ref_x)), rep
+
+
+def test_concurrent_groups_equal_sequential():
+    """Variant groups launched on parallel streams (and captured as parallel
+    graph branches) give the sequential results bit for bit."""
+    xs, _, angs = make_instances(8)
+    ys = [torch.empty((a.numel(), s[4], s[2]), device=DEV) for a, s in zip(angs, SHAPES)]
+    xb = [torch.empty_like(x) for x in xs]
+    pa = batch.GroupedPlan("A", xs, ys, angs)
+    pb = batch.GroupedPlan("AT", ys, xb, angs)
+    pa.launch()
+    pb.launch()
+    torch.cuda.synchronize()
+    ref_y = [t.clone() for t in ys]
+    ref_x = [t.clone() for t in xb]
+    for t in ys + xb:
+        t.fill_(float("nan"))
+    pa.launch(concurrent=True)
+    pb.launch(concurrent=True)
+    torch.cuda.synchronize()
+    assert all(torch.equal(u, v) for u, v in zip(ys, ref_y))
+    assert all(torch.equal(u, v) for u, v in zip(xb, ref_x))
+    graph = batch.capture_pair(pa, pb, concurrent=True)
+    for rep in range(3):
+        for t in ys + xb:
+            t.fill_(float("nan"))
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert all(torch.equal(u, v) for u, v in zip(ys, ref_y)), rep
+        assert all(torch.equal(u, v) for u, v in zip(xb, ref_x)), rep

@@ -217,6 +217,7 @@ class GroupedPlan:
                                 [angles[i] for i in idx], accumulate, v, grid)
                       for v, idx in sorted(groups.items())]
         self.outs = list(outs)
+        self._streams = None
 
     @property
     def work(self):
@@ -226,9 +227,25 @@ class GroupedPlan:
     def variants(self):
         return {p.variant: p.n for p in self.plans}
 
-    def launch(self):
-        for p in self.plans:
-            p.launch()
+    def launch(self, concurrent=False):
+        """One launch per group on the current stream; with concurrent=True
+        the groups run on side streams (default priority) forked from and
+        joined back into the current stream, so the workgroups of one group
+        can fill the CUs left idle by another's tail.  The groups write
+        disjoint outputs, so the results do not change."""
+        if not concurrent or len(self.plans) < 2:
+            for p in self.plans:
+                p.launch()
+            return self.outs
+        cur = torch.cuda.current_stream(self.device)
+        if self._streams is None:
+            self._streams = [torch.cuda.Stream(device=self.device) for _ in self.plans]
+        for p, st in zip(self.plans, self._streams):
+            st.wait_stream(cur)
+            with torch.cuda.stream(st):
+                p.launch()
+        for st in self._streams:
+            cur.wait_stream(st)
         return self.outs
 
 
@@ -238,22 +255,32 @@ def autotune_grouped(gplan, reps=3, persist=True):
 
 
 # --------------------------------------------------------------- graphs
-def capture_pair(plan_a, plan_at, warmup=1):
+def capture_pair(plan_a, plan_at, warmup=1, concurrent=False):
     """Capture `plan_a.launch()` followed by `plan_at.launch()` into a graph.
 
     Returns a torch.cuda.CUDAGraph; replay() re-runs both launches on the
-    same buffers.  Use accumulate=False plans (no fill kernels needed)."""
+    same buffers.  Use accumulate=False plans (no fill kernels needed).  With
+    concurrent=True, GroupedPlans capture their groups as parallel branches
+    (AT still after all of A)."""
+    def run():
+        if concurrent and isinstance(plan_a, GroupedPlan):
+            plan_a.launch(concurrent=True)
+        else:
+            plan_a.launch()
+        if concurrent and isinstance(plan_at, GroupedPlan):
+            plan_at.launch(concurrent=True)
+        else:
+            plan_at.launch()
+
     s = torch.cuda.Stream(device=plan_a.device)
     s.wait_stream(torch.cuda.current_stream(plan_a.device))
     with torch.cuda.stream(s):
         for _ in range(warmup):
-            plan_a.launch()
-            plan_at.launch()
+            run()
     torch.cuda.current_stream(plan_a.device).wait_stream(s)
     g = torch.cuda.CUDAGraph()
     with torch.cuda.graph(g):
-        plan_a.launch()
-        plan_at.launch()
+        run()
     return g
 
 
