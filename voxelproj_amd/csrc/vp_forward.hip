@@ -44,9 +44,11 @@ struct Task {
     int row, j0, j1;
 };
 
-// Footprint slot table of one piece, lane = slot q: t0 the first voxel whose
-// footprint starts at pixel p0 - 2 + q, t1 the second (if any), each as
-// (wl, wr, w3, LDS offset of the voxel's staged slices).
+// Footprint slot table of one piece, lane = slot q: t0 a voxel whose
+// footprint starts at pixel p0 - 2 + q, t1 the other one (if two do), each as
+// (wl, wr, w3, LDS offset of the voxel's staged slices).  Of two such voxels
+// (neighbours on the line), t0 is the one staged first and t1 the next
+// staged voxel, so t1's offset is always t0's plus one voxel.
 struct Slots {
     float4 t0, t1;
 };
@@ -56,6 +58,13 @@ __device__ __forceinline__ int lane_shift_up(int v)
 {
     constexpr int kWaveShr1 = 0x138;
     return __builtin_amdgcn_update_dpp(0, v, kWaveShr1, 0xf, 0xf, false);
+}
+
+// Value of lane + 1 (lane 63: 0), by a DPP wavefront shift.
+__device__ __forceinline__ int lane_shift_down(int v)
+{
+    constexpr int kWaveShl1 = 0x130;
+    return __builtin_amdgcn_update_dpp(0, v, kWaveShl1, 0xf, 0xf, false);
 }
 
 // Cross-lane scatter of (a, b, c, o): lane d/4 of the result receives the
@@ -101,7 +110,10 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     forward_grid<P, NW, HL>(A, H, PX, nbp, nba, nbh);
     static_assert((P + 5) * 1.4143 + 4 <= kSeg, "segment may exceed one wave");
     constexpr int VOX = 64 * HL;                 // floats per staged voxel
-    constexpr int STAGE = UMAX * VOX;            // floats per buffer
+    // floats per buffer: UMAX voxels plus one, so that the sweep's read of a
+    // slot's second voxel (one voxel above the first) stays inside the array
+    // even where the slot has none
+    constexpr int STAGE = (UMAX + 1) * VOX;
     // Staging double buffer as two separate arrays: every access names its
     // buffer statically, so the compiler can tell that the sweep's LDS reads
     // of one buffer do not alias the DMA into the other and does not wait for
@@ -191,6 +203,11 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
 #pragma unroll
             for (int j = 0; j < HL; ++j) part[i][j] = 0.0f;
         folded = true;
+        // Retire the fold's loads and stores here.  Left in flight, they make
+        // the compiler wait on vmcnt before the next sweep's LDS reads, which
+        // also waits for the next piece's DMA issued meanwhile (about 1 % of
+        // A at 512^3 x 192).
+        __builtin_amdgcn_s_waitcnt(0);
     };
 
     for (int pass = 0; pass < 2; ++pass) {
@@ -285,7 +302,9 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
         // Stage a piece straight into LDS with global->LDS DMA: one wave
         // instruction moves one voxel's 64*HL slices (16 B per lane when H is a
         // multiple of 4, else HL dword instructions).  Idle lanes past H load a
-        // clamped, valid address; their results are never stored.
+        // clamped, valid address; their results are never stored.  Staged
+        // slot s holds voxel j0 + s (the order is the same for every wave of
+        // the block, whatever the direction of its angle).
         auto fetch = [&](const Task& t, float4* dst) __attribute__((always_inline)) {
             const int nv = t.j1 - t.j0 + 1;
             const float* base = x + (int64_t)t.row * st_row + (int64_t)t.j0 * st_along;
@@ -343,7 +362,8 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
             // covers l in [p0-2, p0+P-1], the voxels that touch the block's pixels
             const Taps2 tp = anchor_left(
                 voxel_taps(px_ ? (float)jm - xc : R, px_ ? R : (float)jm - yc, k));
-            const int lprev = lane_shift_up(tp.l);  // lane m-1 holds the previous voxel
+            const int lprev = lane_shift_up(tp.l);    // lane m-1 holds the previous voxel
+            const int lnext = lane_shift_down(tp.l);  // lane m+1 the next
             const bool valid = lane < n;
             const int l = valid ? tp.l : 0x3fffffff;
             const int mm0 = uni((int)__popcll(__ballot(valid && l < pl - 1)));
@@ -353,12 +373,15 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
             mA = uni(max(mA, mm0));
             mB = uni(min(mB, mend - 1));
             if (mA > mB) return 0;
-            const int jb0 = fwd ? (jlo - t.j0) : (jhi - t.j0);
+            const int jb0 = fwd ? (jlo - t.j0) : (jhi - t.j0);  // staged slot of lane 0's voxel
             const int dj = fwd ? 1 : -1;
             // Destination lane of each voxel in the rank-0 and rank-1 tables;
             // voxels outside [mA, mB] go to lane 63, which is never a slot.
+            // Of two voxels with the same l (lanes m-1, m), rank 1 is the one
+            // staged later: lane m when j grows with the lane (fwd), else
+            // lane m-1.
             const bool in = lane >= mA && lane <= mB;
-            const bool second = lane > mA && l == lprev;
+            const bool second = fwd ? (lane > mA && l == lprev) : (lane < mB && l == lnext);
             const int q = l - (pl - 1);
             const int d0 = 4 * ((in && !second) ? q : 63);
             const int d1 = 4 * ((in && second) ? q : 63);
@@ -406,10 +429,8 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                 const Mask has3a = (Mask)__ballot(lane < P + 2 && t0.z != 0.f);
                 const Mask has3b = (Mask)__ballot(lane < P + 2 && t1.z != 0.f);
                 const Mask extra = has3a | has2;  // has3b implies has2
-                const int dvox = fwd ? VOX : -VOX;
                 const float* sx = reinterpret_cast<const float*>(xcur) + lane * HL;
-                auto ld = [&](int off) {
-                    const float* src = sx + off;
+                auto ld = [&](const float* src) {
                     struct V { float a[HL]; } out;
                     if constexpr (HL == 4) {
                         const float4 t4 = *reinterpret_cast<const float4*>(src);
@@ -428,16 +449,13 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                     return out;
                 };
                 // Software-pipelined sweep: the two slot rows of pixel q+1 are
-                // fetched (unconditionally, clamped to the staged buffer)
-                // before the FMAs of pixel q.
-                auto slot_offs = [&](int q, int& o0, int& o1) {
-                    o0 = __builtin_bit_cast(int, rdlane(t0.w, q));
-                    o1 = min(max(o0 + dvox, 0), kMaxOff);
-                };
-                int o0, o1;
-                slot_offs(0, o0, o1);
-                auto xa_n = ld(o0);
-                auto xb_n = ld(o1);
+                // fetched (unconditionally) before the FMAs of pixel q.  The
+                // second voxel is the next staged slot, so both reads share one
+                // address and the second takes a constant instruction offset.
+                auto slot_src = [&](int q) { return sx + __builtin_bit_cast(int, rdlane(t0.w, q)); };
+                const float* src = slot_src(0);
+                auto xa_n = ld(src);
+                auto xb_n = ld(src + VOX);
                 // the rank-0 weights of pixel q+1 are read out one pixel ahead too
                 float wl_n = rdlane(t0.x, 0), wr_n = rdlane(t0.y, 0);
 #pragma unroll
@@ -446,9 +464,9 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                     const auto xb = xb_n;
                     const float wl = wl_n, wr = wr_n;
                     if (q + 1 < P + 2) {
-                        slot_offs(q + 1, o0, o1);
-                        xa_n = ld(o0);
-                        xb_n = ld(o1);
+                        src = slot_src(q + 1);
+                        xa_n = ld(src);
+                        xb_n = ld(src + VOX);
                         wl_n = rdlane(t0.x, q + 1);
                         wr_n = rdlane(t0.y, q + 1);
                     }

@@ -224,6 +224,7 @@ class Server:
                     done.append(r)
         ev_end = torch.cuda.Event(enable_timing=True)
         ev_end.record()
+        self._ev_end = ev_end
         self.ticks += 1
         return [(r, ev_plain if r.kind != "SIRT" else ev_end) for r in done]
 
@@ -243,10 +244,6 @@ class Server:
                 raise RuntimeError(f"request {r.rid} needs {r.bytes_needed()} B, over the budget")
         torch.cuda.synchronize(self.device)
         t0 = time.perf_counter()
-        # completion = when the GPU finished the request's last launch, on the
-        # GPU clock relative to this event (recorded on the idle stream at t0)
-        ev0 = torch.cuda.Event(enable_timing=True)
-        ev0.record()
         i = 0
         while i < len(pending) or self.active:
             now = time.perf_counter() - t0
@@ -259,8 +256,13 @@ class Server:
                 continue
             done = self.tick()
             torch.cuda.synchronize(self.device)
+            t_sync = time.perf_counter() - t0
+            # completion on the host clock: the tick's end is observed at
+            # t_sync, and the request's last launch finished earlier by the
+            # GPU time between its event and the tick's end event (never
+            # before the tick was launched, so never before admission)
             for r, ev in done:
-                self._retire(r, ev0.elapsed_time(ev) * 1e-3)
+                self._retire(r, t_sync - ev.elapsed_time(self._ev_end) * 1e-3)
         self.makespan = time.perf_counter() - t0
         return requests
 
