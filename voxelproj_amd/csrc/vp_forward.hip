@@ -319,8 +319,24 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                 const int sub = lane / LPR;
                 const int hoff = min(hb + (lane % LPR) * 4, H - 4);
                 const int nins = (nv + RPI - 1) / RPI;
+                const int nfull = nv / RPI;  // instructions whose RPI voxels all exist
                 const uint32_t range = (uint32_t)(((int64_t)(RPI - 1) * st_along + H) * 4);
-                for (int ins = wave; ins < nins; ins += NW) {
+                // full instructions: the same lane offsets every time, the base
+                // and the LDS destination advance by constant steps
+                const uint32_t voff = (uint32_t)(((int64_t)sub * st_along + hoff) * 4);
+                const int64_t step = (int64_t)NW * RPI * st_along;
+                const float* src = base + (int64_t)wave * RPI * st_along;
+                __attribute__((address_space(3))) float4* d =
+                    (__attribute__((address_space(3))) float4*)&dst[wave * RPI * (VOX / 4)];
+                int ins = wave;
+                for (; ins < nfull; ins += NW) {
+                    dma_to_lds<16>(src, range, voff, (__attribute__((address_space(3))) void*)d);
+                    src += step;
+                    d += NW * RPI * (VOX / 4);
+                }
+                if (ins < nins) {
+                    // the last, partial instruction (RPI > 1 only): lanes past
+                    // the piece load its last voxel again
                     const int v0 = ins * RPI;
                     const int v = min(v0 + sub, nv - 1);
                     dma_to_lds<16>(base + (int64_t)v0 * st_along, range,
