@@ -45,10 +45,10 @@ struct Task {
 };
 
 // Footprint slot table of one piece, lane = slot q: t0 a voxel whose
-// footprint starts at pixel p0 - 2 + q, t1 the other one (if two do), each as
-// (wl, wr, w3, LDS offset of the voxel's staged slices).  Of two such voxels
-// (neighbours on the line), t0 is the one staged first and t1 the next
-// staged voxel, so t1's offset is always t0's plus one voxel.
+// footprint starts at pixel p0 - 2 + q, as (wl, wr, w3, LDS offset of the
+// voxel's staged slices), and t1 the other one (if two do), as (wl, wr, w3,
+// 0).  Of two such voxels (neighbours on the line), t0 is the one staged
+// first and t1 the next staged voxel, so t1's offset is t0's plus one voxel.
 struct Slots {
     float4 t0, t1;
 };
@@ -77,6 +77,17 @@ __device__ __forceinline__ float4 permute4(int d, float a, float b, float c, int
     r.y = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(d, __builtin_bit_cast(int, b)));
     r.z = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(d, __builtin_bit_cast(int, c)));
     r.w = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(d, o));
+    return r;
+}
+
+// The same for (a, b, c) only; the fourth component is zero.
+__device__ __forceinline__ float4 permute3(int d, float a, float b, float c)
+{
+    float4 r;
+    r.x = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(d, __builtin_bit_cast(int, a)));
+    r.y = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(d, __builtin_bit_cast(int, b)));
+    r.z = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(d, __builtin_bit_cast(int, c)));
+    r.w = 0.0f;
     return r;
 }
 
@@ -403,7 +414,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
             const int d1 = 4 * ((in && second) ? q : 63);
             const int off = (jb0 + dj * lane) * VOX;
             st.t0 = permute4(d0, tp.wl, tp.wr, tp.w3, off);
-            st.t1 = permute4(d1, tp.wl, tp.wr, tp.w3, off);
+            st.t1 = permute3(d1, tp.wl, tp.wr, tp.w3);  // its offset is t0's plus one voxel
             return 1;
         };
 
