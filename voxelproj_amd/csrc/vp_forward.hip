@@ -438,10 +438,38 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
             const Task nxt = next_task(cur);
             int flag_n = 0;
             Slots sl_n;
+            // Staging DMA of the next piece.  With one voxel per instruction
+            // (4 slices per lane), each wave issues its instruction k at pixel
+            // slot 2k of its sweep instead of all of them at once right after
+            // the barrier, when every wave of the block queues its DMAs at the
+            // same time (1-4 % faster; same data, same LDS slots).
+            constexpr bool kSpread = (VOX * 4 == 1024);
+            constexpr int kMaxIssue = (UMAX + NW - 1) / NW;
+            constexpr int kInSweep = kMaxIssue < (P + 3) / 2 ? kMaxIssue : (P + 3) / 2;
+            int dcnt = 0;
+            const float* dsrc = x;
+            float4* dd = xnxt;
+            int64_t dstep = 0;
+            uint32_t dvoff = 0, drange = 0;
             if (nxt.row >= 0) {
                 flag_n = setup(nxt, sl_n);
-                fetch(nxt, xnxt);
+                if (kSpread && vec4) {
+                    const int nv = nxt.j1 - nxt.j0 + 1;
+                    dcnt = wave < nv ? (nv - wave + NW - 1) / NW : 0;
+                    dsrc = x + (int64_t)nxt.row * st_row + (int64_t)(nxt.j0 + wave) * st_along;
+                    dd = xnxt + wave * (VOX / 4);
+                    dstep = (int64_t)NW * st_along;
+                    dvoff = (uint32_t)min(hb + lane * 4, H - 4) * 4;
+                    drange = (uint32_t)H * 4;
+                } else {
+                    fetch(nxt, xnxt);
+                }
             }
+            auto issue = [&](int k) __attribute__((always_inline)) {
+                if (k < dcnt)
+                    dma_to_lds<16>(dsrc + k * dstep, drange, dvoff,
+                                   (__attribute__((address_space(3))) void*)(dd + k * NW * (VOX / 4)));
+            };
             if (flag) {
                 constexpr int kMaxOff = (UMAX - 1) * VOX;
                 float4 t0 = sl.t0;
@@ -487,6 +515,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                 float wl_n = rdlane(t0.x, 0), wr_n = rdlane(t0.y, 0);
 #pragma unroll
                 for (int q = 0; q < P + 2; ++q) {
+                    if (q % 2 == 0 && q / 2 < kInSweep) issue(q / 2);
                     const auto xa = xa_n;
                     const auto xb = xb_n;
                     const float wl = wl_n, wr = wr_n;
@@ -529,6 +558,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                     }
                 }
             }
+            for (int k = flag ? kInSweep : 0; k < dcnt; ++k) issue(k);  // not issued in a sweep
             __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA into xnxt has landed
             __syncthreads();
             cur = nxt;
