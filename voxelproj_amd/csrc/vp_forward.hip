@@ -20,7 +20,6 @@
 #include "vp_launch.hpp"
 
 #include <algorithm>
-#include <type_traits>
 
 namespace vp {
 namespace {
@@ -471,18 +470,19 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                                    (__attribute__((address_space(3))) void*)(dd + k * NW * (VOX / 4)));
             };
             if (flag) {
-                constexpr int kMaxOff = (UMAX - 1) * VOX;
-                float4 t0 = sl.t0;
+                const float4 t0 = sl.t0;
                 const float4 t1 = sl.t1;
-                // staged-buffer offsets of the rank-0 voxels, kept in range
-                t0.w = __builtin_bit_cast(float, min(max(__builtin_bit_cast(int, t0.w), 0), kMaxOff));
-                // bit q: slot q holds a second voxel (about 1 in 10 at 512^3 x 192)
-                // (32-bit masks when the slots fit: one scalar bit test each)
-                using Mask = std::conditional_t<(P + 2 <= 32), uint32_t, uint64_t>;
-                const Mask has2 = (Mask)__ballot(lane < P + 2 && (t1.x != 0.f || t1.y != 0.f || t1.z != 0.f));
+                // Slot masks, 32-bit (one scalar bit test each).  Only slot
+                // lanes 0 .. P+1 and the discard lane 63 receive table entries
+                // (the others read zero), and the cast drops lane 63.
+                static_assert(P + 2 <= 32, "slot masks are 32-bit");
+                using Mask = uint32_t;
+                // bit q: slot q holds a second voxel (about 1 in 10 at 512^3 x
+                // 192); a voxel's first weight wl is always positive
+                const Mask has2 = (Mask)__ballot(t1.x != 0.f);
                 // bit q: the rank-0 / rank-1 voxel of slot q covers three pixels
-                const Mask has3a = (Mask)__ballot(lane < P + 2 && t0.z != 0.f);
-                const Mask has3b = (Mask)__ballot(lane < P + 2 && t1.z != 0.f);
+                const Mask has3a = (Mask)__ballot(t0.z != 0.f);
+                const Mask has3b = (Mask)__ballot(t1.z != 0.f);
                 const Mask extra = has3a | has2;  // has3b implies has2
                 const float* sx = reinterpret_cast<const float*>(xcur) + lane * HL;
                 auto ld = [&](const float* src) {
