@@ -111,6 +111,7 @@ class Server:
         self.active = []
         self.in_use = 0
         self.ticks = 0
+        self._ev_end = None  # event after the last launch of the latest tick
 
     # ------------------------------------------------------------ admission
     def can_admit(self, req):
