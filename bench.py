@@ -72,10 +72,41 @@ def time_op(fn, args, n, stream, warmup=3):
     return float(np.median(t)), float(np.percentile(t, 10)), float(np.percentile(t, 90))
 
 
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo), or the platform string."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+def _time_pair(fwd, bwd, budget_s):
+    """Best (total, A, AT) seconds over repetitions of fwd(); bwd() until
+    about budget_s seconds are spent (at least one repetition)."""
+    best, spent, reps = None, 0.0, 0
+    while spent < budget_s or reps < 1:
+        t0 = time.perf_counter()
+        fwd()
+        t1 = time.perf_counter()
+        bwd()
+        t2 = time.perf_counter()
+        spent += t2 - t0
+        reps += 1
+        if best is None or t2 - t0 < best[0]:
+            best = (t2 - t0, t1 - t0, t2 - t1)
+    return best, spent, reps
+
+
 def cpu_baseline(N, A, H_sample, budget_s=10.0):
-    """Reference algorithm (FP32 transcription, OpenMP) on H_sample slices of
-    the workload, repeated until about budget_s seconds of CPU time are spent;
-    the rate is taken from the best repetition."""
+    """The reference algorithm (FP32 transcription, OpenMP) and the exact FP64
+    oracle on H_sample slices of the workload, each repeated for about
+    budget_s seconds of CPU time; rates from the best repetition.  The FP32
+    transcription is the reported value."""
     import oracle as vpo
 
     g = torch.Generator().manual_seed(5)
@@ -83,28 +114,23 @@ def cpu_baseline(N, A, H_sample, budget_s=10.0):
     x = torch.randn((N, N, H_sample), generator=g).numpy()
     y = torch.randn((A, N, H_sample), generator=g).numpy()
     vpo.ref_forward_f32(x[:, :, :1], ang, N, 0)  # warm up threads / pages
-    best = None
-    spent = 0.0
-    reps = 0
-    while spent < budget_s or reps < 1:
-        t0 = time.perf_counter()
-        vpo.ref_forward_f32(x, ang, N, 0)
-        t1 = time.perf_counter()
-        vpo.ref_backward_f32(y, ang, N, N, 0)
-        t2 = time.perf_counter()
-        spent += t2 - t0
-        reps += 1
-        if best is None or t2 - t0 < best[0]:
-            best = (t2 - t0, t1 - t0, t2 - t1)
+    b32, s32, r32 = _time_pair(lambda: vpo.ref_forward_f32(x, ang, N, 0),
+                               lambda: vpo.ref_backward_f32(y, ang, N, N, 0), budget_s)
+    b64, s64, r64 = _time_pair(lambda: vpo.forward_f64(x, ang, N, 0),
+                               lambda: vpo.backward_f64(y, ang, N, N, 0), budget_s)
     work = 2.0 * N * N * H_sample * A
     return {
-        "value": round(work / best[0] / 1e9, 3),
+        "value": round(work / b32[0] / 1e9, 3),
         "unit": "Gvoxel*angle/s",
         "cores": vpo.max_threads(),
         "kind": "port",
-        "sample": (f"A then AT, FP32 transcription of the reference kernels (oracle/), "
-                   f"{N}x{N}x{H_sample} slices x {A} angles (z_order 0), {reps} repetitions "
-                   f"in {spent:.1f} s; best A {best[1]:.2f} s, AT {best[2]:.2f} s"),
+        "cpu_model": cpu_model(),
+        "fp64_oracle_value": round(work / b64[0] / 1e9, 3),
+        "sample": (f"A then AT on {N}x{N}x{H_sample} slices x {A} angles (z_order 0), OpenMP on "
+                   f"{vpo.max_threads()} threads: FP32 transcription of the reference kernels "
+                   f"(oracle/), {r32} repetitions in {s32:.1f} s, best A {b32[1]:.2f} s, AT "
+                   f"{b32[2]:.2f} s; exact FP64 oracle, {r64} repetitions in {s64:.1f} s, best A "
+                   f"{b64[1]:.2f} s, AT {b64[2]:.2f} s"),
     }
 
 

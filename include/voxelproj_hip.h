@@ -93,6 +93,40 @@ VP_API int vp_backward_geom_f64(int z_order, const double *y, double *x, const f
                                 const float *du, const float *dv, double du_max, void *stream);
 
 /*
+ * Workspace.  z_order 2 (and dv) need device memory beyond the inputs and
+ * outputs: the layout transposes and the untranslated sinogram.  The entry
+ * points above take it from a buffer the library keeps per (device, stream),
+ * which grows only when a call needs more, so steady-state calls allocate
+ * nothing; vp_release_workspaces() frees those buffers (it synchronizes the
+ * streams that used them).  vp_forward_ws / vp_backward_ws take the
+ * workspace from the caller instead (for example from a framework's caching
+ * allocator): `workspace` holds `workspace_elems` floats on the current
+ * device, used in stream order on `stream` and free again once the call's
+ * work on that stream has run.  z_order 2 without dv processes H in slabs
+ * when the workspace holds less than vp_workspace_elems(): it needs at least
+ * SY*SX + n_angles*PX floats (one slice).  With dv the full size is needed.
+ * No counterpart in the reference, which allocates a texture array per call
+ * (voxelproj/csrc/projectors.cu:58,102-103).
+ */
+VP_API int vp_forward_ws(int z_order, const float *x, float *y, const float *angles, int n_angles,
+                         int64_t SY, int64_t SX, int64_t H, int64_t PX, const float *du,
+                         const float *dv, double du_max, float *workspace, int64_t workspace_elems,
+                         void *stream);
+VP_API int vp_backward_ws(int z_order, const float *y, float *x, const float *angles, int n_angles,
+                          int64_t SY, int64_t SX, int64_t H, int64_t PX, const float *du,
+                          const float *dv, double du_max, float *workspace, int64_t workspace_elems,
+                          void *stream);
+
+/* Workspace elements (floats, or doubles when f64 != 0) that one call of the
+ * given layout and extents needs without H slabs; 0 when it needs none
+ * (z_order 0 without dv); < 0 on invalid input. */
+VP_API int64_t vp_workspace_elems(int f64, int z_order, int n_angles, int64_t SY, int64_t SX,
+                                  int64_t H, int64_t PX, int with_dv);
+
+/* Free the library's cached workspaces on every device (0 on success). */
+VP_API int vp_release_workspaces(void);
+
+/*
  * The H translation on its own, on slabs of a sinogram, for drivers that
  * stream data larger than device memory through the projectors in H slabs
  * (no counterpart in the reference).  `in` holds global slices

@@ -71,3 +71,17 @@ def test_cu_checks_input_like_reference():
 def test_zorder_error_message():
     err = wrappers.ZOrderError(5)
     assert isinstance(err, KeyError) and isinstance(err, ValueError)
+
+
+def test_reference_import_names_resolve():
+    # `import voxelproj` and `from voxelproj._CU import ...` (the reference's
+    # voxelproj/__init__.py:1-5 and wrappers.py:4-5) resolve to this library.
+    import voxelproj
+    import voxelproj._CU as cu
+    from voxelproj.wrappers import gpu_array_with_default_shape
+
+    assert voxelproj.forward is voxelproj_amd.forward
+    assert voxelproj.backward is voxelproj_amd.backward
+    for name in ("forward_z0", "backward_z0", "forward_z2", "backward_z2"):
+        assert getattr(cu, name) is getattr(_CU, name)
+    assert gpu_array_with_default_shape is wrappers.gpu_array_with_default_shape

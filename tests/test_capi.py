@@ -123,3 +123,33 @@ def test_batched_forward_offset_span_limit():
         assert b"too large for variant" in lib.vp_last_error()
     bad = (_BatchItem * 1)(_BatchItem(1, 1, 1, 1, 1, 4, 2, 1, 5))
     assert lib.vp_forward_batched(1, bad, bad, 1, 0, None) == -1  # accumulate not 0 / 1
+
+
+def test_workspace_sizes():
+    lib = voxelproj_amd.load()
+    A, SY, SX, H, PX = 7, 6, 5, 3, 11
+    nx, ny = SY * SX * H, A * PX * H
+    assert lib.vp_workspace_elems(0, 0, A, SY, SX, H, PX, 0) == 0
+    assert lib.vp_workspace_elems(0, 2, A, SY, SX, H, PX, 0) == nx + ny
+    assert lib.vp_workspace_elems(0, 0, A, SY, SX, H, PX, 1) == ny
+    assert lib.vp_workspace_elems(0, 2, A, SY, SX, H, PX, 1) == nx + 2 * ny
+    assert lib.vp_workspace_elems(1, 2, A, SY, SX, H, PX, 0) == 0
+    assert lib.vp_workspace_elems(1, 0, A, SY, SX, H, PX, 1) == ny
+    assert lib.vp_workspace_elems(0, 1, A, SY, SX, H, PX, 0) < 0
+
+
+def test_caller_workspace_is_validated_before_device_work():
+    lib = voxelproj_amd.load()
+    A, SY, SX, H, PX = 7, 6, 5, 3, 11
+    p = ctypes.c_void_p(1)
+    for fn in (lib.vp_forward_ws, lib.vp_backward_ws):
+        # negative size, or a size without a buffer
+        assert fn(2, p, p, p, A, SY, SX, H, PX, None, None, 0.0, p, -1, None) == -1
+        assert fn(2, p, p, p, A, SY, SX, H, PX, None, None, 0.0, None, 100, None) == -1
+        # z_order 2 needs at least one slice of volume and sinogram
+        one = SY * SX + A * PX
+        assert fn(2, p, p, p, A, SY, SX, H, PX, None, None, 0.0, p, one - 1, None) == -2
+        assert b"workspace" in lib.vp_last_error()
+        # with dv it needs the whole size
+        need = lib.vp_workspace_elems(0, 2, A, SY, SX, H, PX, 1)
+        assert fn(2, p, p, p, A, SY, SX, H, PX, None, p, 0.0, p, need - 1, None) == -2

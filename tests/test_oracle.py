@@ -185,3 +185,52 @@ def test_nnz_per_voxel_angle():
     ang2 = np.array([0.3, 1.2], np.float32)
     cnt = sum(int(np.count_nonzero(_brute.weights(p, 5, 4, 8) > 1e-13)) for p in ang2)
     assert vpo.nnz(ang2, 5, 4, 8) == cnt
+
+
+# ------------------------------------------------ the scalar oracle exports
+def _brute_cdf(phi, t):
+    """Area of the unit voxel centred at 0 whose detector coordinate
+    x cos + y sin is below t (one half-plane clip of the square)."""
+    c, s = np.cos(np.float64(np.float32(phi))), np.sin(np.float64(np.float32(phi)))
+    sq = [(-0.5, -0.5), (0.5, -0.5), (0.5, 0.5), (-0.5, 0.5)]
+    return _brute._area(_brute._clip(sq, -c, -s, -t))
+
+
+@pytest.mark.parametrize("phi", [0.0, 0.2, np.pi / 4, 1.0, np.pi / 2, 2.5])
+def test_cdf_matches_half_plane_area(phi):
+    for t in np.linspace(-1.0, 1.0, 41):
+        assert abs(vpo.cdf_f64(phi, t) - _brute_cdf(phi, t)) < 1e-13, (phi, t)
+
+
+@pytest.mark.parametrize("phi", [0.0, 0.3, np.pi / 4, 1.2, 2.9])
+def test_weight_matches_polygon_area(phi):
+    # one voxel at the volume centre (u = pc): its weights on a 5-pixel
+    # detector are the strip areas of the brute-force definition
+    W = _brute.weights(phi, 1, 1, 5)[:, 0, 0]
+    for p in range(5):
+        assert abs(vpo.weight_f64(phi, p, 2.0) - W[p]) < 1e-13, (phi, p)
+    # and W(p, u) = F(p + 1/2 - u) - F(p - 1/2 - u) for off-centre positions
+    for u in (1.3, 2.45, 2.5, 3.01):
+        for p in range(5):
+            w = vpo.cdf_f64(phi, p + 0.5 - u) - vpo.cdf_f64(phi, p - 0.5 - u)
+            assert abs(vpo.weight_f64(phi, p, u) - w) < 1e-14
+
+
+def test_angle_table_describes_the_footprint():
+    phis = np.array([0.0, 0.1, np.pi / 4, 1.0, np.pi / 2, 2.2, np.pi], np.float32)
+    tab = vpo.angle_table_f64(phis)
+    for (c, s, a, b, h), phi in zip(tab, phis):
+        np.testing.assert_allclose([c, s], [np.cos(np.float64(phi)), np.sin(np.float64(phi))],
+                                   rtol=0, atol=1e-15)
+        # b: support half-width of F; h: its slope on the plateau [-a, a];
+        # unit area: h (a + b) = 1
+        assert vpo.cdf_f64(phi, b) == pytest.approx(1.0, abs=1e-14)
+        assert vpo.cdf_f64(phi, -b) == pytest.approx(0.0, abs=1e-14)
+        assert vpo.cdf_f64(phi, b - 1e-4) < 1.0
+        eps = 1e-7
+        slope = (_brute_cdf(phi, eps) - _brute_cdf(phi, -eps)) / (2 * eps)
+        assert slope == pytest.approx(h, rel=1e-6)
+        assert h * (a + b) == pytest.approx(1.0, abs=1e-14)
+        if a > 1e-3:  # the plateau ends at +-a: F is linear up to a, not beyond
+            mid = _brute_cdf(phi, a / 2)
+            assert mid == pytest.approx(0.5 + h * a / 2, abs=1e-12)

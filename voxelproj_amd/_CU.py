@@ -71,8 +71,17 @@ def ctypes_stream(stream):
 def _call(name, z, inp, out, angles, A, SY, SX, H, PX):
     lib = _lib.load()
     with torch.cuda.device(inp.device):
-        rc = getattr(lib, name)(z, inp.data_ptr(), out.data_ptr(), angles.data_ptr(), A,
-                                SY, SX, H, PX, _stream(inp))
+        stream = _stream(inp)
+        if z == 0:
+            rc = getattr(lib, name)(z, inp.data_ptr(), out.data_ptr(), angles.data_ptr(), A,
+                                    SY, SX, H, PX, stream)
+        else:
+            # z_order 2 runs through layout transposes; their workspace comes
+            # from torch's caching allocator (stream-ordered, reused across calls)
+            ws = _lib.workspace(0, z, A, SY, SX, H, PX, False, inp.device)
+            fn = lib.vp_forward_ws if name == "vp_forward" else lib.vp_backward_ws
+            rc = fn(z, inp.data_ptr(), out.data_ptr(), angles.data_ptr(), A, SY, SX, H, PX,
+                    None, None, 0.0, ws.data_ptr(), ws.numel(), stream)
     _lib.check(rc, name)
 
 

@@ -14,9 +14,10 @@ LIB_PATH = os.path.join(_HERE, "libvoxelproj_hip.so")
 
 EXPORTS = ("vp_forward", "vp_backward", "vp_forward_f64", "vp_backward_f64",
            "vp_forward_geom", "vp_backward_geom", "vp_forward_geom_f64", "vp_backward_geom_f64",
+           "vp_forward_ws", "vp_backward_ws", "vp_workspace_elems", "vp_release_workspaces",
            "vp_vshift", "vp_forward_batched", "vp_backward_batched", "vp_default_variant",
            "vp_batch_blocks", "vp_last_error", "vp_abi_version")
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 _lib = None
 _lock = threading.Lock()
@@ -64,6 +65,16 @@ def load():
             f.argtypes = sig[:-1] + [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                      ctypes.c_void_p]
             f.restype = ctypes.c_int
+        for name in ("vp_forward_ws", "vp_backward_ws"):
+            f = getattr(lib, name)
+            f.argtypes = sig[:-1] + [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                     ctypes.c_void_p, i64, ctypes.c_void_p]
+            f.restype = ctypes.c_int
+        lib.vp_workspace_elems.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, i64, i64, i64,
+                                           i64, ctypes.c_int]
+        lib.vp_workspace_elems.restype = i64
+        lib.vp_release_workspaces.argtypes = []
+        lib.vp_release_workspaces.restype = ctypes.c_int
         lib.vp_vshift.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                   ctypes.c_int, i64, i64, i64, i64, i64, ctypes.c_void_p,
                                   ctypes.c_int, ctypes.c_void_p]
@@ -91,3 +102,15 @@ def check(rc, what):
     if rc != 0:
         msg = load().vp_last_error().decode(errors="replace")
         raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+
+def workspace(f64, z_order, A, SY, SX, H, PX, with_dv, device):
+    """A device workspace tensor of the size one call needs (see
+    vp_workspace_elems), from torch's caching allocator on the current stream."""
+    import torch
+
+    n = int(load().vp_workspace_elems(1 if f64 else 0, z_order, A, SY, SX, H, PX,
+                                      1 if with_dv else 0))
+    if n < 0:
+        raise RuntimeError("invalid extents for a workspace")
+    return torch.empty(max(n, 1), dtype=torch.float64 if f64 else torch.float32, device=device)

@@ -94,8 +94,13 @@ class BatchPlan:
                            1 if accumulate else 0)
             self.shapes.append((int(SY), int(SX), int(H), int(A_), int(PX)))
         self._host = arr
+        # descriptor upload: from pinned memory, stream-ordered, so building a
+        # plan while earlier work is in flight does not wait for the device
         raw = np.frombuffer(bytes(arr), dtype=np.uint8)
-        self._dev = torch.tensor(raw, dtype=torch.uint8, device=self.device)
+        self._pinned = torch.empty(raw.size, dtype=torch.uint8, pin_memory=True)
+        self._pinned.numpy()[:] = raw
+        with torch.cuda.device(self.device):
+            self._dev = self._pinned.to(self.device, non_blocking=True)
         self.n = n
         self.grid = default_grid(self.device) if grid is None else int(grid)
         self.variant = variant if variant is not None else _default_variant(op, self.shapes)

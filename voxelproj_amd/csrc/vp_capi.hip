@@ -3,7 +3,12 @@
 // launched stream-ordered on the caller's stream and every launch is checked.
 #include <stddef.h>
 #include <stdio.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
 #include <string>
+#include <utility>
 
 #include "../../include/voxelproj_hip.h"
 #include "vp_device.hpp"
@@ -89,7 +94,7 @@ int validate_batch(int op, int variant, const vp_batch_item* items, int n)
 
 extern "C" {
 
-int vp_abi_version(void) { return 2; }
+int vp_abi_version(void) { return 3; }
 
 const char* vp_last_error(void) { return g_last_error.c_str(); }
 
@@ -108,153 +113,194 @@ int validate_shift(const float* du, double du_max, vp::Geom* g)
     return VP_OK;
 }
 
-template <typename T>
-struct Workspace {
-    T* p = nullptr;
-    hipStream_t s;
-    hipError_t alloc(int64_t n, hipStream_t st)
-    {
-        s = st;
-        return hipMallocAsync((void**)&p, (size_t)n * sizeof(T), s);
-    }
-    hipError_t release()
-    {
-        hipError_t e = p ? hipFreeAsync(p, s) : hipSuccess;
-        p = nullptr;
-        return e;
-    }
+// ---------------------------------------------------------------- workspace
+// Device memory beyond the inputs and outputs: the layout transposes of
+// z_order 2 and the untranslated sinogram of the dv path.  The caller passes
+// it (vp_forward_ws / vp_backward_ws: memory the caller owns, for instance
+// from its framework's allocator), or the library keeps one buffer per
+// (device, stream) that grows only when a call needs more, so steady-state
+// calls allocate nothing.  The cache's lock is held for the whole call, so a
+// concurrent call on the same stream cannot regrow (free) the buffer between
+// another call's use of it and the enqueueing of that call's kernels.
+struct Span {
+    void* p = nullptr;
+    int64_t bytes = 0;
 };
 
-// y += A x (float32), any layout, optional translations.
-int forward_f32(int z_order, const float* x, float* y, const float* angles, const vp::Geom& g,
-                const float* dv, hipStream_t s)
+std::mutex g_ws_mutex;
+std::map<std::pair<int, hipStream_t>, Span> g_ws;
+
+// Buffer of at least `bytes` for stream s on the current device (caller
+// holds g_ws_mutex).
+hipError_t cached_workspace(int64_t bytes, hipStream_t s, void** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    Span& b = g_ws[{dev, s}];
+    if (b.bytes < bytes) {
+        if (b.p) {
+            e = hipFreeAsync(b.p, s);  // after the stream's earlier uses
+            b.p = nullptr;
+            b.bytes = 0;
+            if (e != hipSuccess) return e;
+        }
+        const int64_t nb = std::max(bytes, (int64_t)256);
+        e = hipMallocAsync(&b.p, (size_t)nb, s);
+        if (e != hipSuccess) {
+            b.p = nullptr;
+            return e;
+        }
+        b.bytes = nb;
+    }
+    *out = b.p;
+    return hipSuccess;
+}
+
+// Workspace (elements of T) that a call needs: z_order 2 transposes the
+// volume and the sinogram (|x| + |y| at full H, or H slabs of a smaller
+// buffer); dv needs the untranslated sinogram (|y|, and with z_order 2 a
+// second one).  Float32 paths only use the z_order 0 kernels; FP64 has
+// native z_order 2 kernels and needs a workspace only for dv.
+int64_t workspace_elems(bool f64, int z_order, const vp::Geom& g, bool dv)
 {
     const int64_t nx = g.SY * g.SX * g.H;
     const int64_t ny = (int64_t)g.A * g.PX * g.H;
-    hipError_t e;
-    if (z_order == 0 && !dv) {
-        e = vp::launch_forward_z0(x, y, angles, g, 1, s);
-        return e == hipSuccess ? VP_OK : hip_fail(e, "forward launch");
+    if (f64) return dv ? ny : 0;
+    if (!dv) return z_order == 2 ? nx + ny : 0;
+    return z_order == 2 ? nx + 2 * ny : ny;
+}
+
+// Minimum workspace of the slab path: one slice of volume and sinogram.
+int64_t slab_min_elems(const vp::Geom& g) { return g.SY * g.SX + (int64_t)g.A * g.PX; }
+
+// out (+)= op(in) in z_order 2 without translations, through H slabs of the
+// workspace (z_order 2 slabs are contiguous): the input slab is transposed
+// into slice-fastest order, the z_order 0 kernel runs on it, and its result
+// is transposed and added into the output slab.  Slab heights are multiples
+// of 256 slices (a whole slice block of every kernel variant) unless the
+// whole H fits or fewer than 256 slices do.
+hipError_t z2_by_slabs(int op, const float* in, float* out, const float* angles,
+                       const vp::Geom& g, float* ws, int64_t ws_elems, hipStream_t s)
+{
+    const int64_t nv = g.SY * g.SX, ns = (int64_t)g.A * g.PX;  // floats per slice
+    int64_t hs = ws_elems / (nv + ns);
+    if (hs >= g.H)
+        hs = g.H;
+    else if (hs >= 256)
+        hs -= hs % 256;
+    hipError_t e = hipSuccess;
+    for (int64_t h0 = 0; h0 < g.H && e == hipSuccess; h0 += hs) {
+        vp::Geom gs = g;
+        gs.H = std::min(hs, g.H - h0);
+        float* wx = ws;                // (SY, SX, hs)
+        float* wy = ws + nv * gs.H;    // (A, PX, hs)
+        if (op == 0) {
+            e = vp::launch_transpose(in + h0 * nv, wx, gs.H, nv, 0, s);
+            if (e == hipSuccess) e = vp::launch_forward_z0(wx, wy, angles, gs, 0, s);
+            if (e == hipSuccess) e = vp::launch_transpose(wy, out + h0 * ns, ns, gs.H, 1, s);
+        } else {
+            e = vp::launch_transpose(in + h0 * ns, wy, gs.H, ns, 0, s);
+            if (e == hipSuccess) e = vp::launch_backward_z0(wy, wx, angles, gs, 0, s);
+            if (e == hipSuccess) e = vp::launch_transpose(wx, out + h0 * nv, nv, gs.H, 1, s);
+        }
     }
-    // Workspace: the volume in slice-fastest order (z_order 2 only), the
-    // untranslated sinogram, and (z_order 2 with dv) the translated one.
-    const int64_t wx = z_order == 2 ? nx : 0;
-    const int64_t wy2 = (z_order == 2 && dv) ? ny : 0;
-    Workspace<float> ws;
-    e = ws.alloc(wx + ny + wy2, s);
-    if (e != hipSuccess) return hip_fail(e, "forward workspace");
+    return e;
+}
+
+// y += A x (float32), any layout, optional translations; ws holds ws_elems
+// floats (see workspace_elems; z_order 2 without dv also runs in slabs).
+hipError_t forward_f32(int z_order, const float* x, float* y, const float* angles,
+                       const vp::Geom& g, const float* dv, float* ws, int64_t ws_elems,
+                       hipStream_t s)
+{
+    const int64_t nx = g.SY * g.SX * g.H;
+    const int64_t ny = (int64_t)g.A * g.PX * g.H;
+    if (!dv) {
+        if (z_order == 0) return vp::launch_forward_z0(x, y, angles, g, 1, s);
+        return z2_by_slabs(0, x, y, angles, g, ws, ws_elems, s);
+    }
+    // dv: the untranslated sinogram y0 (slice-fastest), then its H interpolation
+    hipError_t e = hipSuccess;
     const float* x0 = x;
+    float* y0 = ws;
     if (z_order == 2) {
-        e = vp::launch_transpose(x, ws.p, g.H, g.SY * g.SX, 0, s);
-        x0 = ws.p;
+        e = vp::launch_transpose(x, ws, g.H, g.SY * g.SX, 0, s);
+        x0 = ws;
+        y0 = ws + nx;
     }
-    float* y0 = ws.p + wx;
     if (e == hipSuccess) e = vp::launch_forward_z0(x0, y0, angles, g, 0, s);
     if (z_order == 0) {
         if (e == hipSuccess)
             e = vp::launch_vshift<float>(0, 0, y0, y, g.A, g.PX, g.H, 0, g.H, 0, dv, 1, s);
-    } else {
-        const float* yz = y0;
-        if (dv && e == hipSuccess) {
-            e = vp::launch_vshift<float>(0, 0, y0, y0 + ny, g.A, g.PX, g.H, 0, g.H, 0, dv, 0, s);
-            yz = y0 + ny;
-        }
-        if (e == hipSuccess) e = vp::launch_transpose(yz, y, (int64_t)g.A * g.PX, g.H, 1, s);
+        return e;
     }
-    hipError_t ef = ws.release();
-    if (e != hipSuccess) return hip_fail(e, "forward launch");
-    if (ef != hipSuccess) return hip_fail(ef, "forward workspace release");
-    return VP_OK;
+    if (e == hipSuccess)
+        e = vp::launch_vshift<float>(0, 0, y0, y0 + ny, g.A, g.PX, g.H, 0, g.H, 0, dv, 0, s);
+    if (e == hipSuccess) e = vp::launch_transpose(y0 + ny, y, (int64_t)g.A * g.PX, g.H, 1, s);
+    return e;
 }
 
 // x += AT y (float32), any layout, optional translations.
-int backward_f32(int z_order, const float* y, float* x, const float* angles, const vp::Geom& g,
-                 const float* dv, hipStream_t s)
+hipError_t backward_f32(int z_order, const float* y, float* x, const float* angles,
+                        const vp::Geom& g, const float* dv, float* ws, int64_t ws_elems,
+                        hipStream_t s)
 {
     const int64_t nx = g.SY * g.SX * g.H;
     const int64_t ny = (int64_t)g.A * g.PX * g.H;
-    hipError_t e;
-    if (z_order == 0 && !dv) {
-        e = vp::launch_backward_z0(y, x, angles, g, 1, s);
-        return e == hipSuccess ? VP_OK : hip_fail(e, "backward launch");
+    if (!dv) {
+        if (z_order == 0) return vp::launch_backward_z0(y, x, angles, g, 1, s);
+        return z2_by_slabs(1, y, x, angles, g, ws, ws_elems, s);
     }
-    const int64_t wx = z_order == 2 ? nx : 0;
-    const int64_t wy = ny + ((z_order == 2 && dv) ? ny : 0);
-    Workspace<float> ws;
-    e = ws.alloc(wx + wy, s);
-    if (e != hipSuccess) return hip_fail(e, "backward workspace");
-    float* x0 = ws.p;
-    float* yb = ws.p + wx;
+    // dv: the adjoint H interpolation first, then AT of the untranslated sinogram
+    hipError_t e = hipSuccess;
     const float* yz = y;  // sinogram in slice-fastest order
+    float* x0 = ws;       // z_order 2: the volume before its transpose
+    float* yb = z_order == 2 ? ws + nx : ws;
     if (z_order == 2) {
         e = vp::launch_transpose(y, yb, g.H, (int64_t)g.A * g.PX, 0, s);
         yz = yb;
     }
-    if (dv && e == hipSuccess) {
-        float* g0 = (z_order == 2) ? yb + ny : yb;
-        e = vp::launch_vshift<float>(0, 1, yz, g0, g.A, g.PX, g.H, 0, g.H, 0, dv, 0, s);
-        yz = g0;
-    }
+    float* g0 = z_order == 2 ? yb + ny : yb;
+    if (e == hipSuccess) e = vp::launch_vshift<float>(0, 1, yz, g0, g.A, g.PX, g.H, 0, g.H, 0, dv, 0, s);
     if (z_order == 0) {
-        if (e == hipSuccess) e = vp::launch_backward_z0(yz, x, angles, g, 1, s);
-    } else {
-        if (e == hipSuccess) e = vp::launch_backward_z0(yz, x0, angles, g, 0, s);
-        if (e == hipSuccess) e = vp::launch_transpose(x0, x, g.SY * g.SX, g.H, 1, s);
+        if (e == hipSuccess) e = vp::launch_backward_z0(g0, x, angles, g, 1, s);
+        return e;
     }
-    hipError_t ef = ws.release();
-    if (e != hipSuccess) return hip_fail(e, "backward launch");
-    if (ef != hipSuccess) return hip_fail(ef, "backward workspace release");
-    return VP_OK;
+    if (e == hipSuccess) e = vp::launch_backward_z0(g0, x0, angles, g, 0, s);
+    if (e == hipSuccess) e = vp::launch_transpose(x0, x, g.SY * g.SX, g.H, 1, s);
+    return e;
 }
 
 // y += A x (float64).  With dv the untranslated sinogram goes to a zeroed
 // workspace first (the FP64 kernels accumulate).
-int forward_f64(int z_order, const double* x, double* y, const float* angles, const vp::Geom& g,
-                const float* dv, hipStream_t s)
+hipError_t forward_f64(int z_order, const double* x, double* y, const float* angles,
+                       const vp::Geom& g, const float* dv, double* ws, int64_t /*ws_elems*/,
+                       hipStream_t s)
 {
-    hipError_t e;
-    if (!dv) {
-        e = vp::launch_forward_f64(z_order, x, y, angles, g, s);
-        return e == hipSuccess ? VP_OK : hip_fail(e, "forward f64 launch");
-    }
+    if (!dv) return vp::launch_forward_f64(z_order, x, y, angles, g, s);
     const int64_t ny = (int64_t)g.A * g.PX * g.H;
-    Workspace<double> ws;
-    e = ws.alloc(ny, s);
-    if (e != hipSuccess) return hip_fail(e, "forward f64 workspace");
-    e = vp::launch_fill<double>(ws.p, ny, 0.0, s);
-    if (e == hipSuccess) e = vp::launch_forward_f64(z_order, x, ws.p, angles, g, s);
+    hipError_t e = vp::launch_fill<double>(ws, ny, 0.0, s);
+    if (e == hipSuccess) e = vp::launch_forward_f64(z_order, x, ws, angles, g, s);
     if (e == hipSuccess)
-        e = vp::launch_vshift<double>(z_order, 0, ws.p, y, g.A, g.PX, g.H, 0, g.H, 0, dv, 1, s);
-    hipError_t ef = ws.release();
-    if (e != hipSuccess) return hip_fail(e, "forward f64 launch");
-    if (ef != hipSuccess) return hip_fail(ef, "forward f64 workspace release");
-    return VP_OK;
+        e = vp::launch_vshift<double>(z_order, 0, ws, y, g.A, g.PX, g.H, 0, g.H, 0, dv, 1, s);
+    return e;
 }
 
-int backward_f64(int z_order, const double* y, double* x, const float* angles, const vp::Geom& g,
-                 const float* dv, hipStream_t s)
+hipError_t backward_f64(int z_order, const double* y, double* x, const float* angles,
+                        const vp::Geom& g, const float* dv, double* ws, int64_t /*ws_elems*/,
+                        hipStream_t s)
 {
-    hipError_t e;
-    if (!dv) {
-        e = vp::launch_backward_f64(z_order, y, x, angles, g, s);
-        return e == hipSuccess ? VP_OK : hip_fail(e, "backward f64 launch");
-    }
-    const int64_t ny = (int64_t)g.A * g.PX * g.H;
-    Workspace<double> ws;
-    e = ws.alloc(ny, s);
-    if (e != hipSuccess) return hip_fail(e, "backward f64 workspace");
-    e = vp::launch_vshift<double>(z_order, 1, y, ws.p, g.A, g.PX, g.H, 0, g.H, 0, dv, 0, s);
-    if (e == hipSuccess) e = vp::launch_backward_f64(z_order, ws.p, x, angles, g, s);
-    hipError_t ef = ws.release();
-    if (e != hipSuccess) return hip_fail(e, "backward f64 launch");
-    if (ef != hipSuccess) return hip_fail(ef, "backward f64 workspace release");
-    return VP_OK;
+    if (!dv) return vp::launch_backward_f64(z_order, y, x, angles, g, s);
+    hipError_t e = vp::launch_vshift<double>(z_order, 1, y, ws, g.A, g.PX, g.H, 0, g.H, 0, dv, 0, s);
+    if (e == hipSuccess) e = vp::launch_backward_f64(z_order, ws, x, angles, g, s);
+    return e;
 }
 
 template <typename T, typename Op>
-int run_op(Op op, int z_order, const T* in, T* out, const float* angles, int n_angles, int64_t SY,
-           int64_t SX, int64_t H, int64_t PX, const float* du, const float* dv, double du_max,
-           void* stream)
+int run_op(Op op, const char* what, int z_order, const T* in, T* out, const float* angles,
+           int n_angles, int64_t SY, int64_t SX, int64_t H, int64_t PX, const float* du,
+           const float* dv, double du_max, T* ws, int64_t ws_elems, void* stream)
 {
     g_last_error.clear();
     vp::Geom g;
@@ -263,7 +309,30 @@ int run_op(Op op, int z_order, const T* in, T* out, const float* angles, int n_a
     if (rc != VP_OK || empty) return rc;
     rc = validate_shift(du, du_max, &g);
     if (rc != VP_OK) return rc;
-    return op(z_order, in, out, angles, g, dv, (hipStream_t)stream);
+    if (ws_elems < 0 || (ws_elems > 0 && ws == nullptr))
+        return fail(VP_ERR_ARG, "workspace: negative size or null pointer");
+    const bool f64 = sizeof(T) == 8;
+    const int64_t need = workspace_elems(f64, z_order, g, dv != nullptr);
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e;
+    if (need == 0) {
+        e = op(z_order, in, out, angles, g, dv, nullptr, 0, s);
+    } else if (ws != nullptr) {
+        // a caller's workspace: z_order 2 without dv runs in H slabs of it
+        const int64_t least = (!f64 && !dv) ? slab_min_elems(g) : need;
+        if (ws_elems < least)
+            return fail(VP_ERR_LIMIT, std::string(what) + ": workspace of " + std::to_string(ws_elems) +
+                                          " elements, this call needs at least " + std::to_string(least));
+        e = op(z_order, in, out, angles, g, dv, ws, ws_elems, s);
+    } else {
+        std::lock_guard<std::mutex> lock(g_ws_mutex);
+        void* p = nullptr;
+        e = cached_workspace(need * (int64_t)sizeof(T), s, &p);
+        if (e != hipSuccess) return hip_fail(e, (std::string(what) + " workspace").c_str());
+        e = op(z_order, in, out, angles, g, dv, (T*)p, need, s);
+    }
+    if (e != hipSuccess) return hip_fail(e, (std::string(what) + " launch").c_str());
+    return VP_OK;
 }
 
 }  // namespace
@@ -273,61 +342,113 @@ extern "C" {
 int vp_forward(int z_order, const float* x, float* y, const float* angles, int n_angles,
                int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
 {
-    return run_op<float>(forward_f32, z_order, x, y, angles, n_angles, SY, SX, H, PX, nullptr,
-                         nullptr, 0.0, stream);
+    return run_op<float>(forward_f32, "forward", z_order, x, y, angles, n_angles, SY, SX, H, PX,
+                         nullptr, nullptr, 0.0, nullptr, 0, stream);
 }
 
 int vp_backward(int z_order, const float* y, float* x, const float* angles, int n_angles,
                 int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
 {
-    return run_op<float>(backward_f32, z_order, y, x, angles, n_angles, SY, SX, H, PX, nullptr,
-                         nullptr, 0.0, stream);
+    return run_op<float>(backward_f32, "backward", z_order, y, x, angles, n_angles, SY, SX, H, PX,
+                         nullptr, nullptr, 0.0, nullptr, 0, stream);
 }
 
 int vp_forward_f64(int z_order, const double* x, double* y, const float* angles, int n_angles,
                    int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
 {
-    return run_op<double>(forward_f64, z_order, x, y, angles, n_angles, SY, SX, H, PX, nullptr,
-                          nullptr, 0.0, stream);
+    return run_op<double>(forward_f64, "forward f64", z_order, x, y, angles, n_angles, SY, SX, H,
+                          PX, nullptr, nullptr, 0.0, nullptr, 0, stream);
 }
 
 int vp_backward_f64(int z_order, const double* y, double* x, const float* angles, int n_angles,
                     int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
 {
-    return run_op<double>(backward_f64, z_order, y, x, angles, n_angles, SY, SX, H, PX, nullptr,
-                          nullptr, 0.0, stream);
+    return run_op<double>(backward_f64, "backward f64", z_order, y, x, angles, n_angles, SY, SX, H,
+                          PX, nullptr, nullptr, 0.0, nullptr, 0, stream);
 }
 
 int vp_forward_geom(int z_order, const float* x, float* y, const float* angles, int n_angles,
                     int64_t SY, int64_t SX, int64_t H, int64_t PX, const float* du,
                     const float* dv, double du_max, void* stream)
 {
-    return run_op<float>(forward_f32, z_order, x, y, angles, n_angles, SY, SX, H, PX, du, dv,
-                         du_max, stream);
+    return run_op<float>(forward_f32, "forward", z_order, x, y, angles, n_angles, SY, SX, H, PX,
+                         du, dv, du_max, nullptr, 0, stream);
 }
 
 int vp_backward_geom(int z_order, const float* y, float* x, const float* angles, int n_angles,
                      int64_t SY, int64_t SX, int64_t H, int64_t PX, const float* du,
                      const float* dv, double du_max, void* stream)
 {
-    return run_op<float>(backward_f32, z_order, y, x, angles, n_angles, SY, SX, H, PX, du, dv,
-                         du_max, stream);
+    return run_op<float>(backward_f32, "backward", z_order, y, x, angles, n_angles, SY, SX, H, PX,
+                         du, dv, du_max, nullptr, 0, stream);
 }
 
 int vp_forward_geom_f64(int z_order, const double* x, double* y, const float* angles,
                         int n_angles, int64_t SY, int64_t SX, int64_t H, int64_t PX,
                         const float* du, const float* dv, double du_max, void* stream)
 {
-    return run_op<double>(forward_f64, z_order, x, y, angles, n_angles, SY, SX, H, PX, du, dv,
-                          du_max, stream);
+    return run_op<double>(forward_f64, "forward f64", z_order, x, y, angles, n_angles, SY, SX, H,
+                          PX, du, dv, du_max, nullptr, 0, stream);
 }
 
 int vp_backward_geom_f64(int z_order, const double* y, double* x, const float* angles,
                          int n_angles, int64_t SY, int64_t SX, int64_t H, int64_t PX,
                          const float* du, const float* dv, double du_max, void* stream)
 {
-    return run_op<double>(backward_f64, z_order, y, x, angles, n_angles, SY, SX, H, PX, du, dv,
-                          du_max, stream);
+    return run_op<double>(backward_f64, "backward f64", z_order, y, x, angles, n_angles, SY, SX, H,
+                          PX, du, dv, du_max, nullptr, 0, stream);
+}
+
+int vp_forward_ws(int z_order, const float* x, float* y, const float* angles, int n_angles,
+                  int64_t SY, int64_t SX, int64_t H, int64_t PX, const float* du, const float* dv,
+                  double du_max, float* workspace, int64_t workspace_elems, void* stream)
+{
+    return run_op<float>(forward_f32, "forward", z_order, x, y, angles, n_angles, SY, SX, H, PX,
+                         du, dv, du_max, workspace, workspace_elems, stream);
+}
+
+int vp_backward_ws(int z_order, const float* y, float* x, const float* angles, int n_angles,
+                   int64_t SY, int64_t SX, int64_t H, int64_t PX, const float* du, const float* dv,
+                   double du_max, float* workspace, int64_t workspace_elems, void* stream)
+{
+    return run_op<float>(backward_f32, "backward", z_order, y, x, angles, n_angles, SY, SX, H, PX,
+                         du, dv, du_max, workspace, workspace_elems, stream);
+}
+
+int64_t vp_workspace_elems(int f64, int z_order, int n_angles, int64_t SY, int64_t SX, int64_t H,
+                           int64_t PX, int with_dv)
+{
+    if ((z_order != 0 && z_order != 2) || n_angles < 0 || SY < 0 || SX < 0 || H < 0 || PX < 0)
+        return -1;
+    vp::Geom g;
+    g.SY = SY;
+    g.SX = SX;
+    g.H = H;
+    g.PX = PX;
+    g.A = n_angles;
+    return workspace_elems(f64 != 0, z_order, g, with_dv != 0);
+}
+
+int vp_release_workspaces(void)
+{
+    g_last_error.clear();
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return hip_fail(e, "release workspaces");
+    int rc = VP_OK;
+    for (auto& kv : g_ws) {
+        if (!kv.second.p) continue;
+        e = hipSetDevice(kv.first.first);
+        if (e == hipSuccess) e = hipStreamSynchronize(kv.first.second);
+        if (e == hipSuccess) e = hipFree(kv.second.p);
+        if (e != hipSuccess && rc == VP_OK) rc = hip_fail(e, "release workspaces");
+        kv.second = Span();
+    }
+    g_ws.clear();
+    e = hipSetDevice(cur);
+    if (e != hipSuccess && rc == VP_OK) rc = hip_fail(e, "release workspaces");
+    return rc;
 }
 
 int vp_vshift(int z_order, int transpose, const float* in, float* out, int n_angles, int64_t PX,

@@ -123,17 +123,24 @@ def _launch(kind, z_order, inp, out, angles, A, SY, SX, H, PX, du=None, dv=None)
     lib = _lib.load()
     with torch.cuda.device(inp.device):
         stream = torch.cuda.current_stream(inp.device).cuda_stream
-        if du is None and dv is None:
+        du_p = du.tensor.data_ptr() if du is not None else None
+        dv_p = dv.tensor.data_ptr() if dv is not None else None
+        du_max = du.max_abs if du is not None else 0.0
+        if inp.dtype == torch.float32 and (z_order == 2 or dv is not None):
+            # float32 paths that need a workspace take it from torch's allocator
+            name = "vp_forward_ws" if kind == "A" else "vp_backward_ws"
+            ws = _lib.workspace(False, z_order, A, SY, SX, H, PX, dv is not None, inp.device)
+            rc = getattr(lib, name)(z_order, inp.data_ptr(), out.data_ptr(), angles.data_ptr(),
+                                    A, SY, SX, H, PX, du_p, dv_p, du_max, ws.data_ptr(),
+                                    ws.numel(), stream)
+        elif du is None and dv is None:
             name = _ENTRY[(inp.dtype, kind)]
             rc = getattr(lib, name)(z_order, inp.data_ptr(), out.data_ptr(), angles.data_ptr(),
                                     A, SY, SX, H, PX, stream)
         else:
             name = _ENTRY_GEOM[(inp.dtype, kind)]
             rc = getattr(lib, name)(z_order, inp.data_ptr(), out.data_ptr(), angles.data_ptr(),
-                                    A, SY, SX, H, PX,
-                                    du.tensor.data_ptr() if du is not None else None,
-                                    dv.tensor.data_ptr() if dv is not None else None,
-                                    du.max_abs if du is not None else 0.0, stream)
+                                    A, SY, SX, H, PX, du_p, dv_p, du_max, stream)
     _lib.check(rc, name)
 
 

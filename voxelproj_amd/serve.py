@@ -1,26 +1,35 @@
 """Serving loop: continuous batching of projector and SIRT requests.
 
 Requests (forward A, backprojection AT, or an n-iteration SIRT
-reconstruction) arrive over time.  The server keeps every request's data
-resident on the device and admits a request when its buffers fit the memory
-budget.  It then runs *ticks*.  In one tick every admitted request advances
-by one step:
+reconstruction) arrive over time.  The server admits a request as soon as a
+slot is free and its device buffers fit the memory budget, keeps its data
+resident on the device, and never waits for the device between launches:
 
-  1. one persistent batched A launch over all requests that need an A now
-     (plain A requests, SIRT row sums A1, SIRT A x);
-  2. SIRT residual update r = R (b - A x);
-  3. one persistent batched AT launch over all requests that need an AT
-     (plain AT requests, SIRT column sums AT1, SIRT AT r);
-  4. SIRT update x += C AT r.
+* a request whose own A and AT launches fill the device (at least one
+  workgroup per CU) is enqueued as a whole -- its single launch, or its SIRT
+  chain of 2 + 2 n_iter launches and elementwise steps -- on one of a few
+  request streams, round-robin.  Requests on different streams run
+  concurrently, so a request that arrives while another is on the GPU starts
+  at once, on the CUs that the other's workgroups release;
+* the other requests are batched: on the batching stream, each *tick* is
+  one persistent batched A launch over every such request that needs an A
+  now (plain A requests, SIRT row sums A1, SIRT A x), the SIRT residual
+  updates, one batched AT launch (plain AT requests, SIRT column sums AT1,
+  SIRT AT r) and the SIRT updates; every batched request advances one step
+  per tick.  Requests that arrive while ticks are in flight join the next
+  tick; up to `depth` ticks are queued on the device at a time, and the
+  launch descriptors of an unchanged set of requests are reused.
 
-Requests that arrive while a tick is on the GPU join the next tick, so work
-of different requests overlaps in the same launches ("continuous batching").
-An instance whose own launch fills the device runs as a plain single launch;
-the others run with the kernel variant chosen from their own shape (one
-batched launch per variant group).  The choice depends only on the
-instance's shape, and both paths add the same terms in the same order, so a
-request's result does not depend on what else is in flight: alone and in
-the mix it is bitwise identical.
+The host loop admits, launches and polls GPU events (Event.query); it never
+synchronizes.  A request's completion time is read off the GPU clock: the
+elapsed time from an event recorded at the start of the run to the event
+after the request's last launch.
+
+Which path a request takes depends only on its own shape, and both paths
+add the same terms in the same order (the batched kernels equal the single
+launches bit for bit), so a request's result does not depend on what else is
+in flight: alone and in the mix it is bitwise identical, and equal to the
+plain operators' result.
 
 Latency = completion - arrival; throughput = voxel-angles / makespan, where
 an A or AT of an (SY, SX, H) volume at n_angles counts SY*SX*H*n_angles and a
@@ -28,8 +37,8 @@ SIRT request counts (2 + 2 n_iter) of them.
 
 No counterpart in the reference (a library of two operators).
 """
-import math
 import time
+from collections import OrderedDict
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -56,6 +65,12 @@ class Request:
     t_admit: float = None
     t_done: float = None
     state: dict = field(default_factory=dict)
+
+    def __post_init__(self):
+        if self.kind not in KINDS:
+            raise ValueError(f"request kind must be one of {KINDS}, got {self.kind!r}")
+        if self.kind == "SIRT" and int(self.n_iter) < 0:
+            raise ValueError("a SIRT request needs n_iter >= 0")
 
     @property
     def H(self):
@@ -86,7 +101,8 @@ class Request:
 
 
 def _safe_reciprocal_(t, eps=1e-6):
-    """t <- 1/t where t > eps, else 0 (the SIRT row / column weights)."""
+    """t <- 1/t where t > eps, else 0 (the SIRT row / column weights; the
+    same values as recon.SIRT's)."""
     mask = t > eps
     t.clamp_min_(eps).reciprocal_().mul_(mask)
     return t
@@ -96,10 +112,29 @@ def _variant(op, H):
     return int(_lib.load().vp_default_variant(0 if op == "A" else 1, H))
 
 
-class Server:
-    """Continuous-batching server on one device."""
+def _run_chain(r):
+    """The plain single-instance operators for one request (on the current
+    stream); returns its result."""
+    if r.kind == "A":
+        return ops.project(r.data, r.angles, r.PX)
+    if r.kind == "AT":
+        return ops.backproject(r.data, r.angles, r.SY, r.SX)
+    solver = recon.SIRT(r.angles, r.SY, r.SX, r.H, r.PX, device=r.data.device)
+    x = torch.zeros((r.SY, r.SX, r.H), device=r.data.device)
+    solver.iterate(x, r.data, r.n_iter)
+    return x
 
-    def __init__(self, device="cuda", mem_budget=None, max_active=256, grid=None):
+
+class Server:
+    """Continuous-batching server on one device.
+
+    mem_budget: bytes of request buffers admitted at a time (default 80 % of
+    the free device memory); max_active: request slots; streams: request
+    streams for requests that fill the device; depth: batched ticks queued
+    on the device at a time."""
+
+    def __init__(self, device="cuda", mem_budget=None, max_active=64, grid=None, streams=2,
+                 depth=2):
         self.device = torch.device(device)
         if mem_budget is None:
             free, _total = torch.cuda.mem_get_info(self.device)
@@ -108,77 +143,104 @@ class Server:
         self.max_active = max_active
         self.grid = default_grid(self.device) if grid is None else grid
         self.cus = torch.cuda.get_device_properties(self.device).multi_processor_count
-        self.active = []
+        self.depth = max(1, int(depth))
+        # default-priority streams: one per concurrently running large request, one for ticks
+        self.req_streams = [torch.cuda.Stream(device=self.device) for _ in range(max(1, streams))]
+        self.tick_stream = torch.cuda.Stream(device=self.device)
+        self._next_stream = 0
+        self.active = []        # admitted, not retired
+        self.batched = []       # the active requests served by ticks
         self.in_use = 0
         self.ticks = 0
-        self._ev_end = None  # event after the last launch of the latest tick
+        self._flight = []       # (event, [requests completing at it])
+        self._ticks_in_flight = []  # tick-end events still running
+        self._plans = OrderedDict()  # cached launch descriptors
+        self._ev0 = None
 
     # ------------------------------------------------------------ admission
     def can_admit(self, req):
         return (len(self.active) < self.max_active
                 and self.in_use + req.bytes_needed() <= self.mem_budget)
 
+    def _blocks(self, op, r):
+        code = 0 if op == "A" else 1
+        return _lib.load().vp_batch_blocks(code, _variant(op, r.H), r.n_angles, r.SY, r.SX,
+                                           r.H, r.PX)
+
+    def fills_device(self, req):
+        """Whether the request's own A and AT launches each have at least one
+        workgroup per CU (then batching adds nothing to them)."""
+        ops_needed = {"A": ("A",), "AT": ("AT",), "SIRT": ("A", "AT")}[req.kind]
+        return all(self._blocks(op, req) >= self.cus for op in ops_needed)
+
     def admit(self, req, now):
         req.t_admit = now
         self.in_use += req.bytes_needed()
+        self.active.append(req)
+        main = torch.cuda.current_stream(self.device)
+        if self.fills_device(req):
+            st = self.req_streams[self._next_stream % len(self.req_streams)]
+            self._next_stream += 1
+            st.wait_stream(main)  # the request's data was written on the main stream
+            with torch.cuda.stream(st):
+                req.result = _run_chain(req)
+                ev = torch.cuda.Event(enable_timing=True)
+                ev.record(st)
+            self._flight.append((ev, [req]))
+            return
+        self.tick_stream.wait_stream(main)
         st = req.state
         dev = self.device
-        if req.kind == "A":
-            req.result = torch.empty((req.n_angles, req.PX, req.H), device=dev)
-        elif req.kind == "AT":
-            req.result = torch.empty((req.SY, req.SX, req.H), device=dev)
-        else:
-            vshape = (req.SY, req.SX, req.H)
-            sshape = tuple(req.data.shape)
-            st["x"] = torch.zeros(vshape, device=dev)
-            st["R"] = torch.empty(sshape, device=dev)
-            st["C"] = torch.empty(vshape, device=dev)
-            st["ts"] = torch.empty(sshape, device=dev)
-            st["tv"] = torch.empty(vshape, device=dev)
-            st["ones_v"] = torch.ones(vshape, device=dev)
-            st["ones_s"] = torch.ones(sshape, device=dev)
-            st["it"] = -1  # -1: weights not computed yet
-            req.result = st["x"]
-        self.active.append(req)
+        with torch.cuda.stream(self.tick_stream):
+            if req.kind == "A":
+                req.result = torch.empty((req.n_angles, req.PX, req.H), device=dev)
+            elif req.kind == "AT":
+                req.result = torch.empty((req.SY, req.SX, req.H), device=dev)
+            else:
+                vshape = (req.SY, req.SX, req.H)
+                sshape = tuple(req.data.shape)
+                st["x"] = torch.zeros(vshape, device=dev)
+                st["R"] = torch.empty(sshape, device=dev)
+                st["C"] = torch.empty(vshape, device=dev)
+                st["ts"] = torch.empty(sshape, device=dev)
+                st["tv"] = torch.empty(vshape, device=dev)
+                st["ones_v"] = torch.ones(vshape, device=dev)
+                st["ones_s"] = torch.ones(sshape, device=dev)
+                st["it"] = -1  # -1: weights not computed yet
+                req.result = st["x"]
+        self.batched.append(req)
 
     # ------------------------------------------------------------ one tick
-    def _fills_device(self, op, inp, out, angles, H):
-        """Whether the instance's own launch has at least one workgroup per CU:
-        then it runs as a plain single launch (batching adds nothing to it)."""
-        if op == "A":
-            (SY, SX, _), PX = inp.shape, out.shape[1]
+    def _plan(self, op, v, jobs):
+        """The batched launch of `jobs` [(in, out, angles)] under variant v,
+        built once per set of buffers and reused while that set recurs."""
+        key = (op, v) + tuple((j[0].data_ptr(), j[1].data_ptr(), j[2].data_ptr()) for j in jobs)
+        plan = self._plans.get(key)
+        if plan is None:
+            plan = BatchPlan(op, [j[0] for j in jobs], [j[1] for j in jobs], [j[2] for j in jobs],
+                             accumulate=False, variant=v, grid=self.grid)
+            self._plans[key] = plan
+            while len(self._plans) > 64:
+                self._plans.popitem(last=False)
         else:
-            (SY, SX, _), PX = out.shape, inp.shape[1]
-        code = 0 if op == "A" else 1
-        nb = _lib.load().vp_batch_blocks(code, _variant(op, H), angles.numel(), SY, SX, H, PX)
-        return nb >= self.cus
+            self._plans.move_to_end(key)
+        return plan
 
     def _launch(self, op, jobs):
-        """jobs: list of (in, out, angles, H).  Instances that fill the device
-        on their own run as plain single launches; the rest go into one
-        batched launch per variant.  Both paths add the same terms in the same
-        order, so a result does not depend on which path or batch it took."""
+        """jobs: [(in, out, angles, H)]: one batched launch per kernel variant."""
         groups = {}
         for j in jobs:
-            if self._fills_device(op, *j):
-                inp, out, angles, _H = j
-                out.zero_()
-                if op == "A":
-                    ops.project(inp, angles, out.shape[1], out=out)
-                else:
-                    ops.backproject(inp, angles, out.shape[0], out.shape[1], out=out)
-            else:
-                groups.setdefault(_variant(op, j[3]), []).append(j)
+            groups.setdefault(_variant(op, j[3]), []).append(j)
         for v, js in sorted(groups.items()):
-            BatchPlan(op, [j[0] for j in js], [j[1] for j in js], [j[2] for j in js],
-                      accumulate=False, variant=v, grid=self.grid).launch()
+            self._plan(op, v, js).launch()
 
     def tick(self):
-        """Advance every admitted request by one step.  Single-operator
-        requests are launched first (A, then AT), so they complete before the
-        SIRT steps of the same tick; returns [(request, completion event)]."""
+        """Advance every batched request by one step, on the batching stream,
+        without waiting for the device.  Single-operator requests are launched
+        first (A, then AT), so they complete before the SIRT steps of the same
+        tick; returns [(request, completion event)]."""
         a_plain, at_plain, a_sirt, at_sirt = [], [], [], []
-        for r in self.active:
+        for r in self.batched:
             st = r.state
             if r.kind == "A":
                 a_plain.append((r.data, r.result, r.angles, r.H))
@@ -189,52 +251,80 @@ class Server:
                 at_sirt.append((st["ones_s"], st["C"], r.angles, r.H))
             else:
                 a_sirt.append((st["x"], st["ts"], r.angles, r.H))
-        ev_plain = None
-        if a_plain:
-            self._launch("A", a_plain)
-        if at_plain:
-            self._launch("AT", at_plain)
-        if a_plain or at_plain:
-            ev_plain = torch.cuda.Event(enable_timing=True)
-            ev_plain.record()
-        if a_sirt:
-            self._launch("A", a_sirt)
-        for r in self.active:
-            st = r.state
-            if r.kind == "SIRT":
-                if st["it"] < 0:
-                    _safe_reciprocal_(st["R"])
-                else:
-                    torch.sub(r.data, st["ts"], out=st["ts"])
-                    st["ts"].mul_(st["R"])
-                    at_sirt.append((st["ts"], st["tv"], r.angles, r.H))
-        if at_sirt:
-            self._launch("AT", at_sirt)
         done = []
-        for r in self.active:
-            st = r.state
-            if r.kind != "SIRT":
-                done.append(r)
-            elif st["it"] < 0:
-                _safe_reciprocal_(st["C"])
-                st["it"] = 0
-            else:
-                st["x"].addcmul_(st["C"], st["tv"])
-                st["it"] += 1
-                if st["it"] >= r.n_iter:
+        with torch.cuda.stream(self.tick_stream):
+            ev_plain = None
+            if a_plain:
+                self._launch("A", a_plain)
+            if at_plain:
+                self._launch("AT", at_plain)
+            if a_plain or at_plain:
+                ev_plain = torch.cuda.Event(enable_timing=True)
+                ev_plain.record(self.tick_stream)
+            if a_sirt:
+                self._launch("A", a_sirt)
+            for r in self.batched:
+                st = r.state
+                if r.kind == "SIRT":
+                    if st["it"] < 0:
+                        _safe_reciprocal_(st["R"])
+                    else:
+                        torch.sub(r.data, st["ts"], out=st["ts"])
+                        st["ts"].mul_(st["R"])
+                        at_sirt.append((st["ts"], st["tv"], r.angles, r.H))
+            if at_sirt:
+                self._launch("AT", at_sirt)
+            for r in self.batched:
+                st = r.state
+                if r.kind != "SIRT":
                     done.append(r)
-        ev_end = torch.cuda.Event(enable_timing=True)
-        ev_end.record()
-        self._ev_end = ev_end
+                elif st["it"] < 0:
+                    _safe_reciprocal_(st["C"])
+                    st["it"] = 0
+                    if r.n_iter <= 0:
+                        done.append(r)
+                else:
+                    st["x"].addcmul_(st["C"], st["tv"])
+                    st["it"] += 1
+                    if st["it"] >= r.n_iter:
+                        done.append(r)
+            ev_end = torch.cuda.Event(enable_timing=True)
+            ev_end.record(self.tick_stream)
         self.ticks += 1
+        for r in done:
+            self.batched.remove(r)
+        plain = [r for r in done if r.kind != "SIRT"]
+        sirt = [r for r in done if r.kind == "SIRT"]
+        if plain:
+            self._flight.append((ev_plain, plain))
+        if sirt:
+            self._flight.append((ev_end, sirt))
+        self._ticks_in_flight.append(ev_end)
         return [(r, ev_plain if r.kind != "SIRT" else ev_end) for r in done]
 
-    def _retire(self, r, now):
-        r.t_done = now
+    def _retire(self, r, t_done):
+        r.t_done = t_done
         self.in_use -= r.bytes_needed()
         for k in ("R", "C", "ts", "tv", "ones_v", "ones_s"):
             r.state.pop(k, None)
         self.active.remove(r)
+
+    def _poll(self):
+        """Retire the requests whose completion events have fired; returns
+        whether any did."""
+        fired = False
+        keep = []
+        for ev, reqs in self._flight:
+            if ev.query():
+                t = self._ev0.elapsed_time(ev) * 1e-3
+                for r in reqs:
+                    self._retire(r, max(t, r.t_admit))
+                fired = True
+            else:
+                keep.append((ev, reqs))
+        self._flight = keep
+        self._ticks_in_flight = [e for e in self._ticks_in_flight if not e.query()]
+        return fired
 
     # ------------------------------------------------------------ run
     def run(self, requests):
@@ -243,7 +333,10 @@ class Server:
         for r in pending:
             if r.bytes_needed() > self.mem_budget:
                 raise RuntimeError(f"request {r.rid} needs {r.bytes_needed()} B, over the budget")
+        main = torch.cuda.current_stream(self.device)
         torch.cuda.synchronize(self.device)
+        self._ev0 = torch.cuda.Event(enable_timing=True)
+        self._ev0.record(main)
         t0 = time.perf_counter()
         i = 0
         while i < len(pending) or self.active:
@@ -251,19 +344,20 @@ class Server:
             while i < len(pending) and pending[i].arrival <= now and self.can_admit(pending[i]):
                 self.admit(pending[i], now)
                 i += 1
+            if self.batched and len(self._ticks_in_flight) < self.depth:
+                self.tick()
+                continue
+            if self._poll():
+                continue
             if not self.active:
                 # idle: wait for the next arrival
                 time.sleep(max(0.0, pending[i].arrival - now))
                 continue
-            done = self.tick()
-            torch.cuda.synchronize(self.device)
-            t_sync = time.perf_counter() - t0
-            # completion on the host clock: the tick's end is observed at
-            # t_sync, and the request's last launch finished earlier by the
-            # GPU time between its event and the tick's end event (never
-            # before the tick was launched, so never before admission)
-            for r, ev in done:
-                self._retire(r, t_sync - ev.elapsed_time(self._ev_end) * 1e-3)
+            # work is in flight: look again shortly (new arrivals, completions)
+            time.sleep(5e-5)
+        torch.cuda.synchronize(self.device)
+        for st in self.req_streams + [self.tick_stream]:
+            main.wait_stream(st)
         self.makespan = time.perf_counter() - t0
         return requests
 
@@ -277,14 +371,7 @@ def run_alone(req, device="cuda"):
 
 
 def _run_one_plain(r):
-    if r.kind == "A":
-        return ops.project(r.data, r.angles, r.PX)
-    if r.kind == "AT":
-        return ops.backproject(r.data, r.angles, r.SY, r.SX)
-    solver = recon.SIRT(r.angles, r.SY, r.SX, r.H, r.PX, device=r.data.device)
-    x = torch.zeros((r.SY, r.SX, r.H), device=r.data.device)
-    solver.iterate(x, r.data, r.n_iter)
-    return x
+    return _run_chain(r)
 
 
 def run_plain(requests):
