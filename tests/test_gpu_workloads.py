@@ -98,7 +98,7 @@ def test_least_squares_gradient_at_256(least_squares_256):
     loss2, g = autograd.least_squares_gradient(x.detach(), b, ang, N)
     # the autograd backward is the same hand-written AT of the same residual
     assert torch.equal(x.grad, g)
-    assert float(loss) == pytest.approx(float(loss2), rel=1e-5)
+    assert float(loss.detach()) == pytest.approx(float(loss2), rel=1e-5)
     _check_backward_slices(ops.project(x.detach(), ang, N) - b, ang, g, "gradient")
 
 
