@@ -127,6 +127,25 @@ VP_API int64_t vp_workspace_elems(int f64, int z_order, int n_angles, int64_t SY
 VP_API int vp_release_workspaces(void);
 
 /*
+ * Allocation-free projector call, for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  No counterpart in the reference.  op 0: out += A in
+ * (in = volume, out = sinogram); op 1: out += AT in.  The same kernels and
+ * semantics as vp_forward / vp_backward; z_order 2 runs its layout
+ * transposes through H slabs of a static device scratch array of
+ * vp_noalloc_scratch_elems() floats (64 MiB per device, held until the
+ * process ends), so it needs SY*SX + n_angles*PX <= that size and otherwise
+ * returns VP_ERR_LIMIT.  Calls that overlap on one device share the scratch
+ * and race: serialize them (one stream, or events).
+ */
+VP_API int vp_project_noalloc(int op, int z_order, const float *in, float *out,
+                              const float *angles, int n_angles, int64_t SY, int64_t SX,
+                              int64_t H, int64_t PX, void *stream);
+
+/* Floats in vp_project_noalloc's scratch. */
+VP_API int64_t vp_noalloc_scratch_elems(void);
+
+/*
  * The H translation on its own, on slabs of a sinogram, for drivers that
  * stream data larger than device memory through the projectors in H slabs
  * (no counterpart in the reference).  `in` holds global slices

@@ -153,3 +153,19 @@ def test_caller_workspace_is_validated_before_device_work():
         # with dv it needs the whole size
         need = lib.vp_workspace_elems(0, 2, A, SY, SX, H, PX, 1)
         assert fn(2, p, p, p, A, SY, SX, H, PX, None, p, 0.0, p, need - 1, None) == -2
+
+
+def test_noalloc_validation_before_device_work():
+    lib = voxelproj_amd.load()
+    p = ctypes.c_void_p(1)
+    n = lib.vp_noalloc_scratch_elems()
+    assert n == 16 << 20
+    # bad op, bad z_order
+    assert lib.vp_project_noalloc(2, 0, p, p, p, 3, 4, 4, 2, 5, None) == -1
+    assert lib.vp_project_noalloc(0, 1, p, p, p, 3, 4, 4, 2, 5, None) == -1
+    # z_order 2 with one slice larger than the scratch
+    side = 4097
+    assert lib.vp_project_noalloc(1, 2, p, p, p, 1, side, side, 1, side, None) == -2
+    assert b"scratch" in lib.vp_last_error()
+    # empty problems are a no-op
+    assert lib.vp_project_noalloc(0, 2, p, p, p, 0, 4, 4, 2, 5, None) == 0

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libvoxelproj_hip.so")
 EXPORTS = ("vp_forward", "vp_backward", "vp_forward_f64", "vp_backward_f64",
            "vp_forward_geom", "vp_backward_geom", "vp_forward_geom_f64", "vp_backward_geom_f64",
            "vp_forward_ws", "vp_backward_ws", "vp_workspace_elems", "vp_release_workspaces",
+           "vp_project_noalloc", "vp_noalloc_scratch_elems",
            "vp_vshift", "vp_forward_batched", "vp_backward_batched", "vp_default_variant",
            "vp_batch_blocks", "vp_last_error", "vp_abi_version")
 ABI_VERSION = 3
@@ -73,6 +74,10 @@ def load():
         lib.vp_workspace_elems.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, i64, i64, i64,
                                            i64, ctypes.c_int]
         lib.vp_workspace_elems.restype = i64
+        lib.vp_project_noalloc.argtypes = [ctypes.c_int] + sig
+        lib.vp_project_noalloc.restype = ctypes.c_int
+        lib.vp_noalloc_scratch_elems.argtypes = []
+        lib.vp_noalloc_scratch_elems.restype = i64
         lib.vp_release_workspaces.argtypes = []
         lib.vp_release_workspaces.restype = ctypes.c_int
         lib.vp_vshift.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,

@@ -335,6 +335,40 @@ int run_op(Op op, const char* what, int z_order, const T* in, T* out, const floa
     return VP_OK;
 }
 
+// ------------------------------------------------ allocation-free variant
+// Scratch of vp_project_noalloc: one file-scope device array, allocated
+// with the code object on each device the library runs on and held until
+// the process ends.  Host code never uses the array's name as an address:
+// its device address is looked up once per device (hipGetSymbolAddress,
+// with that device current) and kept.  Every call on a device shares that
+// device's array, so calls that overlap on one device (different streams,
+// or a captured graph replayed beside a call) race on it.  Nothing in it
+// needs to start at zero: each slab's transposes and kernel overwrite what
+// they later read.
+constexpr int64_t kScratchElems = (int64_t)16 << 20;  // 16 Mi floats = 64 MiB
+__device__ __align__(256) float scratch[kScratchElems];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+hipError_t scratch_of_current_device(float** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+
 }  // namespace
 
 extern "C" {
@@ -450,6 +484,37 @@ int vp_release_workspaces(void)
     if (e != hipSuccess && rc == VP_OK) rc = hip_fail(e, "release workspaces");
     return rc;
 }
+
+int vp_project_noalloc(int op, int z_order, const float* in, float* out, const float* angles,
+                       int n_angles, int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
+{
+    g_last_error.clear();
+    if (op != 0 && op != 1) return fail(VP_ERR_ARG, "op must be 0 (A) or 1 (AT), got " + std::to_string(op));
+    vp::Geom g;
+    bool empty = false;
+    int rc = validate(z_order, in, out, angles, n_angles, SY, SX, H, PX, &g, &empty);
+    if (rc != VP_OK || empty) return rc;
+    if (z_order == 2 && slab_min_elems(g) > kScratchElems)
+        return fail(VP_ERR_LIMIT, "vp_project_noalloc: one slice of volume and sinogram (SY*SX + "
+                                  "n_angles*PX = " + std::to_string(slab_min_elems(g)) +
+                                  " floats) exceeds the scratch of " + std::to_string(kScratchElems) +
+                                  " floats");
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e;
+    if (z_order == 0) {
+        e = op == 0 ? vp::launch_forward_z0(in, out, angles, g, 1, s)
+                    : vp::launch_backward_z0(in, out, angles, g, 1, s);
+    } else {
+        float* ws = nullptr;
+        e = scratch_of_current_device(&ws);
+        if (e != hipSuccess) return hip_fail(e, "vp_project_noalloc scratch");
+        e = z2_by_slabs(op, in, out, angles, g, ws, kScratchElems, s);
+    }
+    if (e != hipSuccess) return hip_fail(e, "vp_project_noalloc launch");
+    return VP_OK;
+}
+
+int64_t vp_noalloc_scratch_elems(void) { return kScratchElems; }
 
 int vp_vshift(int z_order, int transpose, const float* in, float* out, int n_angles, int64_t PX,
               int64_t H_in, int64_t in_offset, int64_t H_out, int64_t out_offset,
