@@ -31,7 +31,7 @@ import torch
 from . import _lib, ops, recon
 
 FORWARD_VARIANTS = (1, 2, 4)
-BACKWARD_VARIANTS = (1, 2)
+BACKWARD_VARIANTS = (1, 2, 4)
 
 
 class _Item(ctypes.Structure):

@@ -323,6 +323,295 @@ __device__ __forceinline__ void backproject_block(const OpArgs& g, const int log
     }
 }
 
+// 4 slices per lane.  Block = TX x TY voxel tile x 256 slices in NW waves of
+// VPW voxels; lane = 4 consecutive slices, so each tap is one 16-byte LDS
+// read per lane (ds_read_b128, per byte the fastest LDS read) and each
+// wave-uniform weight feeds 4 FMAs.  Footprints are evaluated once per group
+// of APL = 64 / VPW angles (lane = (voxel, angle)), anchored at their first
+// nonzero pixel: two taps always, the third (about one voxel-angle in eight
+// at 512^3 x 192) behind a wave-uniform bit test of a ballot, which leaves
+// every sum unchanged (a skipped tap has weight 0).  Each group is consumed
+// over NSUB window chunks of AC angles; the window double buffer is two
+// separately declared LDS arrays and the chunk loop is unrolled over the
+// group, so every tap read names its buffer statically and the DMA into the
+// other buffer does not hold it up.  The DMA rows of a group's chunks are
+// located once per group, lane-parallel, so issuing a row costs one readlane
+// and scalar address arithmetic.  Sums are two-level: per group of APL angles
+// in registers, then across groups.
+template <int TX, int TY, int NW, int AC, int WIN>
+__host__ __device__ inline void backward4_grid(int64_t SY, int64_t SX, int64_t H, int& nbx, int& nby,
+                                               int& nbh)
+{
+    nbx = (int)((SX + TX - 1) / TX);
+    nby = (int)((SY + TY - 1) / TY);
+    nbh = (int)((H + 255) / 256);
+}
+
+template <int TX, int TY, int NW, int AC, int WIN>
+__device__ __forceinline__ void backproject4_block(const OpArgs& g, const int logical)
+{
+    constexpr int HL = 4;
+    constexpr int VPW = TX * TY / NW;      // voxels per wave
+    constexpr int APL = 64 / VPW;          // angles per footprint group
+    constexpr int NSUB = APL / AC;         // window chunks per group
+    constexpr int ROW = 64 * HL;           // floats per staged pixel row
+    constexpr int STAGE = AC * WIN * ROW;  // floats per window buffer
+    constexpr int RPW = (AC * WIN + NW - 1) / NW;  // DMA rows per wave and chunk (at most)
+    static_assert(VPW * APL == 64 && APL % AC == 0 && NSUB % 2 == 0, "lane layout");
+    static_assert(NSUB * RPW <= 64, "DMA rows of a group must fit one lane each");
+    // Window coverage: over the tile, r - min r <= floor(L) + 1 with
+    // L = sqrt((TX-1)^2 + (TY-1)^2); the first tap l is r - 1 or r, and the
+    // third tap is read only when l = r - 1, so the rows read are at most
+    // r + 1 - rlo <= floor(L) + 4 <= WIN - 1 (rlo = min r - 2).
+    static_assert(isqrt((TX - 1) * (TX - 1) + (TY - 1) * (TY - 1)) + 5 <= WIN,
+                  "sinogram window too small for the tile");
+    __shared__ float4 s_wa[STAGE / 4];
+    __shared__ float4 s_wb[STAGE / 4];
+    constexpr int kTabAngles = 256;
+    static_assert(kTabAngles % APL == 0 && kTabAngles <= 64 * NW, "angle table range");
+    __shared__ AngleK s_k[2][kTabAngles];
+    __shared__ int s_rlo[2][kTabAngles];
+
+    const float* __restrict__ y = g.in;
+    float* __restrict__ x = g.out;
+    const float* __restrict__ angles = g.angles;
+    const int A = g.A, SY = g.SY, SX = g.SX, H = g.H, PX = g.PX;
+    const double scale = g.scale, inv_scale = g.inv_scale;
+    int nbx, nby, nbh_;
+    backward4_grid<TX, TY, NW, AC, WIN>(SY, SX, H, nbx, nby, nbh_);
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = uni(tid >> 6);
+    const int bx = logical % nbx;
+    const int by = (logical / nbx) % nby;
+    const int bhc = logical / (nbx * nby);
+    const int tx0 = bx * TX, ty0 = by * TY;
+    const int hb = bhc * ROW;
+    const float xc = 0.5f * (float)(SX - 1);
+    const float yc = 0.5f * (float)(SY - 1);
+    const bool vec4 = (H & 3) == 0;
+
+    // footprint lane layout: voxel lv of this wave, angle la of the group
+    const int lv = lane % VPW;
+    const int la = lane / VPW;
+    const int vi = wave * VPW + lv;
+    const float X = (float)(tx0 + vi % TX) - xc;
+    const float Y = (float)(ty0 + vi / TX) - yc;
+
+    auto tables = [&](int a0, int tb) {
+        if (tid < kTabAngles && a0 + tid < A) {
+            const int a = a0 + tid;
+            const AngleK k = make_angle(angles[a], g.du ? g.du[a] : 0.0f, 0.5 * (double)(PX - 1), scale, inv_scale);
+            s_k[tb][tid] = k;
+            const float xa = (float)tx0 - xc;
+            const float xb = (float)(min(tx0 + TX, SX) - 1) - xc;
+            const float ya = (float)ty0 - yc;
+            const float yb = (float)(min(ty0 + TY, SY) - 1) - yc;
+            int r = voxel_r(xa, ya, k);
+            r = min(r, voxel_r(xb, ya, k));
+            r = min(r, voxel_r(xa, yb, k));
+            r = min(r, voxel_r(xb, yb, k));
+            s_rlo[tb][tid] = r - 2;
+        }
+    };
+    auto tab_of = [&](int a) { return (a / kTabAngles) & 1; };
+    auto ent_of = [&](int a) { return a % kTabAngles; };
+    // Sinogram row (angle * PX + pixel) of DMA row `row` (= a * WIN + k) of
+    // chunk `chunk`: pixel rlo + k of the chunk's angle a, clamped to the
+    // detector and to the last angle (taps there carry zero weight).
+    auto dma_row = [&](int chunk, int row) {
+        const int ag = min(chunk * AC + row / WIN, A - 1);
+        const int p = min(max(s_rlo[tab_of(ag)][ent_of(ag)] + row % WIN, 0), PX - 1);
+        return (uint32_t)ag * (uint32_t)PX + (uint32_t)p;
+    };
+    // DMA of one staged row (sinogram row srow) into row `row` of dst; float q
+    // of a staged row holds slice hb + q.
+    const uint32_t voff4 = (uint32_t)min(hb + lane * 4, H - 4) * 4u;
+    auto issue_row = [&](uint32_t srow, int row, float4* dst) __attribute__((always_inline)) {
+        const float* base = y + (int64_t)srow * H;
+        if (vec4) {
+            dma_to_lds<16>(base, (uint32_t)H * 4u, voff4,
+                           (__attribute__((address_space(3))) void*)(dst + row * (ROW / 4)));
+        } else {
+#pragma unroll
+            for (int c4 = 0; c4 < HL; ++c4) {
+                const uint32_t hh = (uint32_t)min(hb + 64 * c4 + lane, H - 1);
+                dma_to_lds<4>(base, (uint32_t)H * 4u, hh * 4u,
+                              (__attribute__((address_space(3))) void*)(reinterpret_cast<float*>(dst + row * (ROW / 4)) + 64 * c4));
+            }
+        }
+    };
+    // DMA of a whole chunk (the first one; later chunks go through the rows
+    // located per group)
+    auto fetch = [&](int chunk, float4* dst) {
+        for (int row = wave; row < AC * WIN; row += NW) issue_row((uint32_t)uni((int)dma_row(chunk, row)), row, dst);
+    };
+
+    float acc[VPW][HL];
+#pragma unroll
+    for (int v = 0; v < VPW; ++v)
+#pragma unroll
+        for (int j = 0; j < HL; ++j) acc[v][j] = 0.0f;
+
+    const int nch = (A + AC - 1) / AC;  // chunks holding at least one angle
+    const int ngroups = (A + APL - 1) / APL;
+    tables(0, 0);
+    __syncthreads();
+    fetch(0, s_wa);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+
+    for (int gi = 0; gi < ngroups; ++gi) {
+        const int a0 = gi * APL;
+        const int tb = tab_of(a0);
+        // The last group of a table range builds the next range's table first
+        // (into the other buffer, idle since the previous range), so that the
+        // DMA rows of the next group's first chunk can be located below.
+        if ((a0 + APL) % kTabAngles == 0 && a0 + APL < A) {
+            tables(a0 + APL, tab_of(a0 + APL));
+            __syncthreads();
+        }
+        // footprint of (voxel lv, angle a0 + la); the group lies in one table range
+        const int al = a0 + la;
+        const bool aval = al < A;
+        const int ae = ent_of(min(al, A - 1));
+        const AngleK k = s_k[tb][ae];
+        const Taps2 t = anchor_left(voxel_taps(X, Y, k));
+        const float wl = (aval && t.l >= 0 && t.l < PX) ? t.wl : 0.0f;
+        const float wr = (aval && t.l + 1 >= 0 && t.l + 1 < PX) ? t.wr : 0.0f;
+        const float w3 = (aval && t.l + 2 >= 0 && t.l + 2 < PX) ? t.w3 : 0.0f;
+        const uint64_t has3 = __ballot(w3 != 0.0f);
+        // Window row of pixel l.  For the tile's voxels the clamp never acts:
+        // l - rlo <= floor(L) + 3 <= WIN - 2, and <= WIN - 3 when the third tap
+        // is used (l = r - 1); it keeps the reads of voxels past the volume's
+        // edge (never stored) inside their angle's window.
+        int off = t.l - s_rlo[tb][ae];
+        off = min(max(off, 0), w3 != 0.0f ? WIN - 3 : WIN - 2);
+        const int foff = ((la % AC) * WIN + off) * ROW;  // float offset of pixel l in its buffer
+        // DMA rows of the chunks that this group's sub-chunks prefetch (c + 1
+        // for c = gi * NSUB + sub): lane sub * RPW + j holds row wave + j * NW.
+        uint32_t rows_l = 0;
+        {
+            const int sub = lane / RPW, j = lane % RPW;
+            const int row = wave + j * NW;
+            if (lane < NSUB * RPW && row < AC * WIN) rows_l = dma_row(gi * NSUB + sub + 1, row);
+        }
+
+        float part[VPW][HL];
+#pragma unroll
+        for (int v = 0; v < VPW; ++v)
+#pragma unroll
+            for (int j = 0; j < HL; ++j) part[v][j] = 0.0f;
+
+#pragma unroll
+        for (int sub = 0; sub < NSUB; ++sub) {
+            const int c = gi * NSUB + sub;
+            if (c >= nch) break;  // block-uniform
+            if (c + 1 < nch) {
+                float4* dst = (sub & 1) ? s_wa : s_wb;
+#pragma unroll
+                for (int j = 0; j < RPW; ++j) {
+                    const int row = wave + j * NW;
+                    if (j * NW < AC * WIN - NW + 1 || row < AC * WIN)
+                        issue_row((uint32_t)__builtin_amdgcn_readlane((int)rows_l, sub * RPW + j), row, dst);
+                }
+            }
+            const float* sw = reinterpret_cast<const float*>((sub & 1) ? s_wb : s_wa) + lane * HL;
+            struct T2 {
+                float4 t[2];
+            };
+            auto tap_ptr = [&](int L) { return sw + __builtin_amdgcn_readlane(foff, L); };
+            auto load2 = [&](const float* pr) {
+                T2 o;
+                o.t[0] = *reinterpret_cast<const float4*>(pr);
+                o.t[1] = *reinterpret_cast<const float4*>(pr + ROW);
+                return o;
+            };
+            constexpr int NI = AC * VPW;  // voxel-angles of this chunk per wave
+            const int L0 = sub * NI;      // lanes L0 .. L0 + NI - 1 hold them
+            const float* pn = tap_ptr(L0);
+            T2 nxt = load2(pn);
+#pragma unroll
+            for (int i = 0; i < NI; ++i) {
+                const int v = i % VPW;
+                const int L = L0 + i;
+                const float* pc = pn;
+                const T2 cur = nxt;
+                if (i + 1 < NI) {
+                    pn = tap_ptr(L + 1);
+                    nxt = load2(pn);
+                }
+                const float c0 = rdlane(wl, L);
+                const float c1 = rdlane(wr, L);
+                part[v][0] = fmaf(c0, cur.t[0].x, part[v][0]);
+                part[v][1] = fmaf(c0, cur.t[0].y, part[v][1]);
+                part[v][2] = fmaf(c0, cur.t[0].z, part[v][2]);
+                part[v][3] = fmaf(c0, cur.t[0].w, part[v][3]);
+                part[v][0] = fmaf(c1, cur.t[1].x, part[v][0]);
+                part[v][1] = fmaf(c1, cur.t[1].y, part[v][1]);
+                part[v][2] = fmaf(c1, cur.t[1].z, part[v][2]);
+                part[v][3] = fmaf(c1, cur.t[1].w, part[v][3]);
+                if ((has3 >> L) & 1ull) {
+                    const float c2 = rdlane(w3, L);
+                    const float4 t3 = *reinterpret_cast<const float4*>(pc + 2 * ROW);
+                    part[v][0] = fmaf(c2, t3.x, part[v][0]);
+                    part[v][1] = fmaf(c2, t3.y, part[v][1]);
+                    part[v][2] = fmaf(c2, t3.z, part[v][2]);
+                    part[v][3] = fmaf(c2, t3.w, part[v][3]);
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA into the other buffer landed
+            __syncthreads();
+        }
+#pragma unroll
+        for (int v = 0; v < VPW; ++v)
+#pragma unroll
+            for (int j = 0; j < HL; ++j) acc[v][j] += part[v][j];
+    }
+
+    // Write-out: the old values of the wave's voxels are loaded first (clamped
+    // addresses), then all stores; 16-byte accesses where the lane's 4 slices
+    // exist and H is a multiple of 4.
+    const int hh0 = hb + lane * HL;
+    const bool full = vec4 && hh0 + HL <= H;
+    float4 old[VPW];
+    int64_t base[VPW];
+    bool vin[VPW];
+#pragma unroll
+    for (int v = 0; v < VPW; ++v) {
+        const int wv = wave * VPW + v;
+        const int ix = tx0 + wv % TX;
+        const int iy = ty0 + wv / TX;
+        vin[v] = ix < SX && iy < SY;
+        base[v] = vin[v] ? ((int64_t)iy * SX + ix) * H : 0;
+    }
+    if (full) {
+#pragma unroll
+        for (int v = 0; v < VPW; ++v)
+            old[v] = g.accumulate ? *reinterpret_cast<const float4*>(x + base[v] + hh0)
+                                  : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int v = 0; v < VPW; ++v) {
+            if (vin[v])
+                *reinterpret_cast<float4*>(x + base[v] + hh0) =
+                    make_float4(old[v].x + acc[v][0], old[v].y + acc[v][1], old[v].z + acc[v][2],
+                                old[v].w + acc[v][3]);
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < VPW; ++v) {
+            if (vin[v]) {
+#pragma unroll
+                for (int j = 0; j < HL; ++j) {
+                    const int hh = hh0 + j;
+                    if (hh < H) x[base[v] + hh] = (g.accumulate ? x[base[v] + hh] : 0.0f) + acc[v][j];
+                }
+            }
+        }
+    }
+}
+
 // Block order: consecutive workgroups (round-robin over the 8 XCDs) are
 // neighbouring tiles of one slice chunk, so concurrently running tiles read
 // overlapping sinogram rows, served from the shared Infinity Cache.
@@ -373,6 +662,67 @@ __global__ __launch_bounds__(64 * NW) void backproject_batched_kernel(
         backproject_block<TX, TY, NW, AC, WIN, HL>(item_args(items[i]), (int)(w - base));
         __syncthreads();
     }
+}
+
+// 4 slices per lane: 16 x 8 voxels in 16 waves of 8 (4 waves per SIMD),
+// windows of 21 pixels in chunks of 2 angles (86 KB of double-buffered LDS),
+// footprints in groups of 8 angles.
+struct Bwd4Cfg {
+    static constexpr int TX = 16, TY = 8, NW = 16, AC = 2, WIN = 21;
+};
+
+template <int TX, int TY, int NW, int AC, int WIN>
+__global__ __launch_bounds__(64 * NW) void backproject4_z0_kernel(OpArgs g)
+{
+    backproject4_block<TX, TY, NW, AC, WIN>(g, blockIdx.x);
+}
+
+template <int TX, int TY, int NW, int AC, int WIN>
+__global__ __launch_bounds__(64 * NW) void backproject4_batched_kernel(
+    const BatchItem* __restrict__ items, int n, int64_t total)
+{
+    for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
+        int i = 0;
+        int64_t base = 0;
+        for (; i < n - 1; ++i) {
+            int nbx, nby, nbh;
+            backward4_grid<TX, TY, NW, AC, WIN>(items[i].SY, items[i].SX, items[i].H, nbx, nby, nbh);
+            const int64_t nbl = (int64_t)nbx * nby * nbh;
+            if (w < base + nbl) break;
+            base += nbl;
+        }
+        backproject4_block<TX, TY, NW, AC, WIN>(item_args(items[i]), (int)(w - base));
+        __syncthreads();
+    }
+}
+
+hipError_t backward4_single(const OpArgs& g, hipStream_t s)
+{
+    using C = Bwd4Cfg;
+    int nbx, nby, nbh;
+    backward4_grid<C::TX, C::TY, C::NW, C::AC, C::WIN>(g.SY, g.SX, g.H, nbx, nby, nbh);
+    const int64_t nb = (int64_t)nbx * nby * nbh;
+    if (nb > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((backproject4_z0_kernel<C::TX, C::TY, C::NW, C::AC, C::WIN>),
+                       dim3((unsigned)nb), dim3(64 * C::NW), 0, s, g);
+    return hipGetLastError();
+}
+
+hipError_t backward4_batched(const BatchItem* host, const BatchItem* dev, int n, int grid,
+                             hipStream_t s)
+{
+    using C = Bwd4Cfg;
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        int nbx, nby, nbh;
+        backward4_grid<C::TX, C::TY, C::NW, C::AC, C::WIN>(host[i].SY, host[i].SX, host[i].H, nbx, nby, nbh);
+        total += (int64_t)nbx * nby * nbh;
+    }
+    if (total == 0) return hipSuccess;
+    const int64_t gsz = grid > 0 ? std::min<int64_t>(grid, total) : total;
+    hipLaunchKernelGGL((backproject4_batched_kernel<C::TX, C::TY, C::NW, C::AC, C::WIN>),
+                       dim3((unsigned)gsz), dim3(64 * C::NW), 0, s, dev, n, total);
+    return hipGetLastError();
 }
 
 // Tile configurations.  2 slices per lane: 16 x 16 voxels and 16 waves of 16
@@ -433,8 +783,7 @@ hipError_t backward_batched(const BatchItem* host, const BatchItem* dev, int n, 
 
 int default_variant(int op, int64_t H)
 {
-    if (op == 0) return H > 128 ? 4 : (H > 64 ? 2 : 1);
-    return H > 64 ? 2 : 1;
+    return H > 128 ? 4 : (H > 64 ? 2 : 1);
 }
 
 int device_cus()
@@ -455,7 +804,7 @@ int device_cus()
 int single_variant(int op, int A, int64_t SY, int64_t SX, int64_t H, int64_t PX)
 {
     const int v = default_variant(op, H);
-    if (v != 2) return v;
+    if (v != 2) return v;  // 4 slices per lane is kept: it adds terms in its own order
     const int cus = device_cus();
     if (op == 0) return forward_blocks(2, A, H, PX) < cus ? 1 : 2;
     // AT: with fewer 2-slice workgroups than CUs, 1 slice per lane.  Else
@@ -489,7 +838,9 @@ hipError_t launch_backward_z0(const float* y, float* x, const float* angles, con
     a.scale = ldexp(1.0, g.bits);
     a.inv_scale = ldexp(1.0, -g.bits);
     a.du = g.du;
-    if (single_variant(1, g.A, g.SY, g.SX, g.H, g.PX) == 2) return backward_single<2>(a, s);
+    const int v = single_variant(1, g.A, g.SY, g.SX, g.H, g.PX);
+    if (v == 4) return backward4_single(a, s);
+    if (v == 2) return backward_single<2>(a, s);
     if (backward_blocks(1, g.SY, g.SX, g.H) < device_cus()) return backward_launch<BwdSmall, 1>(a, s);
     return backward_single<1>(a, s);
 }
@@ -500,6 +851,7 @@ hipError_t launch_backward_batched(int variant, const BatchItem* items_host,
     switch (variant) {
         case 1: return backward_batched<1>(items_host, items_dev, n, grid, s);
         case 2: return backward_batched<2>(items_host, items_dev, n, grid, s);
+        case 4: return backward4_batched(items_host, items_dev, n, grid, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -510,6 +862,7 @@ int64_t backward_blocks(int variant, int64_t SY, int64_t SX, int64_t H)
     switch (variant) {
         case 1: backward_grid<BwdCfg<1>::TX, BwdCfg<1>::TY, 1>(SY, SX, H, nbx, nby, nbh); break;
         case 2: backward_grid<BwdCfg<2>::TX, BwdCfg<2>::TY, 2>(SY, SX, H, nbx, nby, nbh); break;
+        case 4: backward4_grid<Bwd4Cfg::TX, Bwd4Cfg::TY, Bwd4Cfg::NW, Bwd4Cfg::AC, Bwd4Cfg::WIN>(SY, SX, H, nbx, nby, nbh); break;
         default: return -1;
     }
     return (int64_t)nbx * nby * nbh;

@@ -35,7 +35,7 @@ struct BatchItem {
 
 // Slices per lane of the kernel variants (the autotuner's search space).
 constexpr int kForwardVariants[] = {1, 2, 4};
-constexpr int kBackwardVariants[] = {1, 2};
+constexpr int kBackwardVariants[] = {1, 2, 4};
 int default_variant(int op, int64_t H);  // op 0 = A, 1 = AT
 // Variant of a single (non-batched) launch: the H-based default, except that
 // 2 slices per lane drops to 1 for A when its grid has fewer workgroups than
