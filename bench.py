@@ -227,8 +227,10 @@ def bench_serving(dev, n=40, rate=200.0, seed=0):
     from voxelproj_amd import serve
 
     reqs = serve.mixed_workload(n, rate=rate, seed=seed, device=dev)
-    warm = serve.mixed_workload(4, rate=rate, seed=seed + 1, sizes=(64,), heights=(64,),
-                                n_angles=(90,), device=dev)
+    # warm-up: every kernel configuration the mix can use (small batched and
+    # single launches at 1, 2 and 4 slices per lane)
+    warm = serve.mixed_workload(12, rate=rate, seed=seed + 1, sizes=(64, 256), heights=(64, 128, 256),
+                                n_angles=(90,), n_iter=1, device=dev)
     serve.Server(device=dev).run(warm)
     server = serve.Server(device=dev)
     server.run(reqs)

@@ -3,33 +3,34 @@
 Requests (forward A, backprojection AT, or an n-iteration SIRT
 reconstruction) arrive over time.  The server admits a request as soon as a
 slot is free and its device buffers fit the memory budget, keeps its data
-resident on the device, and never waits for the device between launches:
+resident on the device, and splits its work into *steps*: one projector
+launch each (with the SIRT elementwise updates that follow it), so a plain
+request is one step and an n-iteration SIRT request 2 + 2 n steps.
 
-* a request whose own A and AT launches fill the device (at least one
-  workgroup per CU) is enqueued as a whole -- its single launch, or its SIRT
-  chain of 2 + 2 n_iter launches and elementwise steps -- on one of a few
-  request streams, round-robin.  Requests on different streams run
-  concurrently, so a request that arrives while another is on the GPU starts
-  at once, on the CUs that the other's workgroups release;
-* the other requests are batched: on the batching stream, each *tick* is
-  one persistent batched A launch over every such request that needs an A
-  now (plain A requests, SIRT row sums A1, SIRT A x), the SIRT residual
-  updates, one batched AT launch (plain AT requests, SIRT column sums AT1,
-  SIRT AT r) and the SIRT updates; every batched request advances one step
-  per tick.  Requests that arrive while ticks are in flight join the next
-  tick; up to `depth` ticks are queued on the device at a time, and the
-  launch descriptors of an unchanged set of requests are reused.
+Steps go to device streams, each queued ahead of the device up to about
+`queue_ms` of estimated GPU time (at least 2 and at most `depth` steps, so
+the next step is already queued when one ends), and the host never waits
+for the device: it admits new arrivals, polls GPU events for completions
+and, whenever a stream's queue has room, submits the next step of that
+stream's request with the least remaining work (shortest remaining work
+first, ties in arrival order).  A request whose launches fill the device (at
+least one workgroup per CU) is pinned to one of two request streams, the one
+with less remaining work; the others share a third stream, where the next
+steps of all of them with the same operator run as one persistent batched
+launch (one per kernel variant; the launch descriptors of a recurring set of
+requests are reused; a lone small request runs as a single launch in its
+own tuned configuration).  Kernels of different streams run concurrently and
+fill each other's idle CUs; a request that arrives while others are on the
+GPU starts as soon as the step in flight on its stream ends, and a short
+request overtakes a long SIRT chain ("continuous batching").
+A request's completion time is read off the GPU
+clock: the elapsed time from an event recorded at the start of the run to
+the event after its last step.
 
-The host loop admits, launches and polls GPU events (Event.query); it never
-synchronizes.  A request's completion time is read off the GPU clock: the
-elapsed time from an event recorded at the start of the run to the event
-after the request's last launch.
-
-Which path a request takes depends only on its own shape, and both paths
-add the same terms in the same order (the batched kernels equal the single
-launches bit for bit), so a request's result does not depend on what else is
-in flight: alone and in the mix it is bitwise identical, and equal to the
-plain operators' result.
+Which path a request takes depends only on its own shape, and the batched
+kernels add every output's terms in the same order as the single launches,
+so a request's result does not depend on what else is in flight: alone and
+in the mix it is bitwise identical, and equal to the plain operators'.
 
 Latency = completion - arrival; throughput = voxel-angles / makespan, where
 an A or AT of an (SY, SX, H) volume at n_angles counts SY*SX*H*n_angles and a
@@ -38,7 +39,7 @@ SIRT request counts (2 + 2 n_iter) of them.
 No counterpart in the reference (a library of two operators).
 """
 import time
-from collections import OrderedDict
+from collections import OrderedDict, deque
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -129,12 +130,14 @@ class Server:
     """Continuous-batching server on one device.
 
     mem_budget: bytes of request buffers admitted at a time (default 80 % of
-    the free device memory); max_active: request slots; streams: request
-    streams for requests that fill the device; depth: batched ticks queued
-    on the device at a time."""
+    the free device memory); max_active: request slots; queue_ms / depth:
+    estimated GPU time and number of steps queued on the device at most (a
+    step's time is estimated from its voxel-angles at `est_rate`; the queue
+    always holds two steps when there is work).  Short queues let a new short request
+    overtake sooner; long ones hide the host's submission time."""
 
-    def __init__(self, device="cuda", mem_budget=None, max_active=64, grid=None, streams=2,
-                 depth=2):
+    def __init__(self, device="cuda", mem_budget=None, max_active=64, grid=None, depth=16,
+                 queue_ms=2.0, est_rate=5e12):
         self.device = torch.device(device)
         if mem_budget is None:
             free, _total = torch.cuda.mem_get_info(self.device)
@@ -144,16 +147,16 @@ class Server:
         self.grid = default_grid(self.device) if grid is None else grid
         self.cus = torch.cuda.get_device_properties(self.device).multi_processor_count
         self.depth = max(1, int(depth))
-        # default-priority streams: one per concurrently running large request, one for ticks
-        self.req_streams = [torch.cuda.Stream(device=self.device) for _ in range(max(1, streams))]
-        self.tick_stream = torch.cuda.Stream(device=self.device)
-        self._next_stream = 0
-        self.active = []        # admitted, not retired
-        self.batched = []       # the active requests served by ticks
+        self.queue_ms = float(queue_ms)
+        self.est_rate = float(est_rate)
+        # default-priority streams: two for requests that fill the device, one
+        # for the batched small requests
+        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(3)]
+        self.active = []         # admitted, not retired (arrival order)
         self.in_use = 0
-        self.ticks = 0
-        self._flight = []       # (event, [requests completing at it])
-        self._ticks_in_flight = []  # tick-end events still running
+        self.ticks = 0           # steps submitted (batched steps count once)
+        self._flight = []        # (event, [requests completing at it])
+        self._queued = [[], [], []]  # per stream: (event, estimated ms) of steps still queued or running
         self._plans = OrderedDict()  # cached launch descriptors
         self._ev0 = None
 
@@ -174,43 +177,55 @@ class Server:
         return all(self._blocks(op, req) >= self.cus for op in ops_needed)
 
     def admit(self, req, now):
+        """Allocate the request's buffers (stream-ordered after the data it
+        was given) and queue its steps: (op, input, output, post), where post
+        is the elementwise work that follows the launch."""
         req.t_admit = now
         self.in_use += req.bytes_needed()
-        self.active.append(req)
-        main = torch.cuda.current_stream(self.device)
-        if self.fills_device(req):
-            st = self.req_streams[self._next_stream % len(self.req_streams)]
-            self._next_stream += 1
-            st.wait_stream(main)  # the request's data was written on the main stream
-            with torch.cuda.stream(st):
-                req.result = _run_chain(req)
-                ev = torch.cuda.Event(enable_timing=True)
-                ev.record(st)
-            self._flight.append((ev, [req]))
-            return
-        self.tick_stream.wait_stream(main)
         st = req.state
+        small = not self.fills_device(req)
+        if small:
+            lane = 2
+        else:  # the request stream with less remaining work
+            left = [sum(r.state["remaining"] for r in self.active if r.state.get("lane") == k)
+                    for k in (0, 1)]
+            lane = 0 if left[0] <= left[1] else 1
+        self.active.append(req)
+        stream = self.streams[lane]
+        stream.wait_stream(torch.cuda.current_stream(self.device))
         dev = self.device
-        with torch.cuda.stream(self.tick_stream):
+        steps = deque()
+        with torch.cuda.stream(stream):
             if req.kind == "A":
                 req.result = torch.empty((req.n_angles, req.PX, req.H), device=dev)
+                steps.append(("A", req.data, req.result, None))
             elif req.kind == "AT":
                 req.result = torch.empty((req.SY, req.SX, req.H), device=dev)
+                steps.append(("AT", req.data, req.result, None))
             else:
                 vshape = (req.SY, req.SX, req.H)
                 sshape = tuple(req.data.shape)
-                st["x"] = torch.zeros(vshape, device=dev)
-                st["R"] = torch.empty(sshape, device=dev)
-                st["C"] = torch.empty(vshape, device=dev)
-                st["ts"] = torch.empty(sshape, device=dev)
-                st["tv"] = torch.empty(vshape, device=dev)
-                st["ones_v"] = torch.ones(vshape, device=dev)
-                st["ones_s"] = torch.ones(sshape, device=dev)
-                st["it"] = -1  # -1: weights not computed yet
-                req.result = st["x"]
-        self.batched.append(req)
+                x = torch.zeros(vshape, device=dev)
+                R = torch.empty(sshape, device=dev)
+                C = torch.empty(vshape, device=dev)
+                ts = torch.empty(sshape, device=dev)
+                tv = torch.empty(vshape, device=dev)
+                st.update(x=x, R=R, C=C, ts=ts, tv=tv, ones_v=torch.ones(vshape, device=dev),
+                          ones_s=torch.ones(sshape, device=dev))
+                req.result = x
+                b = req.data
+                # the same elementwise steps as recon.SIRT
+                steps.append(("A", st["ones_v"], R, lambda: _safe_reciprocal_(R)))
+                steps.append(("AT", st["ones_s"], C, lambda: _safe_reciprocal_(C)))
+                for _ in range(int(req.n_iter)):
+                    steps.append(("A", x, ts, lambda: (torch.sub(b, ts, out=ts), ts.mul_(R))))
+                    steps.append(("AT", ts, tv, lambda: x.addcmul_(C, tv)))
+        st["steps"] = steps
+        st["small"] = small
+        st["lane"] = lane
+        st["remaining"] = req.unit_work() * len(steps)
 
-    # ------------------------------------------------------------ one tick
+    # ------------------------------------------------------------ launches
     def _plan(self, op, v, jobs):
         """The batched launch of `jobs` [(in, out, angles)] under variant v,
         built once per set of buffers and reused while that set recurs."""
@@ -226,86 +241,67 @@ class Server:
             self._plans.move_to_end(key)
         return plan
 
-    def _launch(self, op, jobs):
-        """jobs: [(in, out, angles, H)]: one batched launch per kernel variant."""
-        groups = {}
-        for j in jobs:
-            groups.setdefault(_variant(op, j[3]), []).append(j)
-        for v, js in sorted(groups.items()):
-            self._plan(op, v, js).launch()
+    def _launch_single(self, r, op, inp, out):
+        out.zero_()
+        if op == "A":
+            ops.project(inp, r.angles, out.shape[1], out=out)
+        else:
+            ops.backproject(inp, r.angles, out.shape[0], out.shape[1], out=out)
 
-    def tick(self):
-        """Advance every batched request by one step, on the batching stream,
-        without waiting for the device.  Single-operator requests are launched
-        first (A, then AT), so they complete before the SIRT steps of the same
-        tick; returns [(request, completion event)]."""
-        a_plain, at_plain, a_sirt, at_sirt = [], [], [], []
-        for r in self.batched:
-            st = r.state
-            if r.kind == "A":
-                a_plain.append((r.data, r.result, r.angles, r.H))
-            elif r.kind == "AT":
-                at_plain.append((r.data, r.result, r.angles, r.H))
-            elif st["it"] < 0:
-                a_sirt.append((st["ones_v"], st["R"], r.angles, r.H))
-                at_sirt.append((st["ones_s"], st["C"], r.angles, r.H))
-            else:
-                a_sirt.append((st["x"], st["ts"], r.angles, r.H))
+    def tick(self, lane=None):
+        """Submit one step on one stream (by default the first stream whose
+        queue has room and work): the next step of its request with the least
+        remaining work, batched with the next steps of the stream's other
+        small requests of the same operator.  Nothing waits for the device.
+        Returns the requests whose last step it was."""
+        if lane is None:
+            lanes = [k for k in range(3) if self._ready(k) and self._queue_has_room(k)]
+            if not lanes:
+                return []
+            lane = lanes[0]
+        ready = self._ready(lane)
+        if not ready:
+            return []
+        lead = min(ready, key=lambda r: r.state["remaining"])  # ties: arrival order
+        op = lead.state["steps"][0][0]
+        if lead.state["small"]:
+            group = [r for r in ready if r.state["steps"][0][0] == op]
+        else:
+            group = [lead]
         done = []
-        with torch.cuda.stream(self.tick_stream):
-            ev_plain = None
-            if a_plain:
-                self._launch("A", a_plain)
-            if at_plain:
-                self._launch("AT", at_plain)
-            if a_plain or at_plain:
-                ev_plain = torch.cuda.Event(enable_timing=True)
-                ev_plain.record(self.tick_stream)
-            if a_sirt:
-                self._launch("A", a_sirt)
-            for r in self.batched:
-                st = r.state
-                if r.kind == "SIRT":
-                    if st["it"] < 0:
-                        _safe_reciprocal_(st["R"])
-                    else:
-                        torch.sub(r.data, st["ts"], out=st["ts"])
-                        st["ts"].mul_(st["R"])
-                        at_sirt.append((st["ts"], st["tv"], r.angles, r.H))
-            if at_sirt:
-                self._launch("AT", at_sirt)
-            for r in self.batched:
-                st = r.state
-                if r.kind != "SIRT":
+        stream = self.streams[lane]
+        with torch.cuda.stream(stream):
+            steps = [r.state["steps"].popleft() for r in group]
+            if len(group) > 1:
+                by_variant = {}
+                for r, (_, inp, out, _post) in zip(group, steps):
+                    by_variant.setdefault(_variant(op, r.H), []).append((inp, out, r.angles))
+                for v, jobs in sorted(by_variant.items()):
+                    self._plan(op, v, jobs).launch()
+            else:
+                _, inp, out, _post = steps[0]
+                self._launch_single(lead, op, inp, out)
+            for r, (_, _inp, _out, post) in zip(group, steps):
+                if post is not None:
+                    post()
+                r.state["remaining"] -= r.unit_work()
+                if not r.state["steps"]:
                     done.append(r)
-                elif st["it"] < 0:
-                    _safe_reciprocal_(st["C"])
-                    st["it"] = 0
-                    if r.n_iter <= 0:
-                        done.append(r)
-                else:
-                    st["x"].addcmul_(st["C"], st["tv"])
-                    st["it"] += 1
-                    if st["it"] >= r.n_iter:
-                        done.append(r)
-            ev_end = torch.cuda.Event(enable_timing=True)
-            ev_end.record(self.tick_stream)
+            ev = torch.cuda.Event(enable_timing=True)
+            ev.record(stream)
         self.ticks += 1
-        for r in done:
-            self.batched.remove(r)
-        plain = [r for r in done if r.kind != "SIRT"]
-        sirt = [r for r in done if r.kind == "SIRT"]
-        if plain:
-            self._flight.append((ev_plain, plain))
-        if sirt:
-            self._flight.append((ev_end, sirt))
-        self._ticks_in_flight.append(ev_end)
-        return [(r, ev_plain if r.kind != "SIRT" else ev_end) for r in done]
+        self._queued[lane].append((ev, sum(r.unit_work() for r in group) / self.est_rate * 1e3))
+        if done:
+            self._flight.append((ev, done))
+        return done
+
+    def _ready(self, lane):
+        return [r for r in self.active if r.state.get("lane") == lane and r.state.get("steps")]
 
     def _retire(self, r, t_done):
         r.t_done = t_done
         self.in_use -= r.bytes_needed()
-        for k in ("R", "C", "ts", "tv", "ones_v", "ones_s"):
+        for k in ("R", "C", "ts", "tv", "ones_v", "ones_s", "steps", "lane"):
             r.state.pop(k, None)
         self.active.remove(r)
 
@@ -323,8 +319,13 @@ class Server:
             else:
                 keep.append((ev, reqs))
         self._flight = keep
-        self._ticks_in_flight = [e for e in self._ticks_in_flight if not e.query()]
+        self._queued = [[(e, ms) for e, ms in q if not e.query()] for q in self._queued]
         return fired
+
+    def _queue_has_room(self, lane):
+        # two steps at least, so the next one is already queued when one ends
+        q = self._queued[lane]
+        return len(q) < 2 or (len(q) < self.depth and sum(ms for _, ms in q) < self.queue_ms)
 
     # ------------------------------------------------------------ run
     def run(self, requests):
@@ -344,19 +345,20 @@ class Server:
             while i < len(pending) and pending[i].arrival <= now and self.can_admit(pending[i]):
                 self.admit(pending[i], now)
                 i += 1
-            if self.batched and len(self._ticks_in_flight) < self.depth:
+            self._poll()
+            if any(self._ready(k) and self._queue_has_room(k) for k in range(3)):
                 self.tick()
                 continue
-            if self._poll():
-                continue
             if not self.active:
+                if i == len(pending):
+                    break
                 # idle: wait for the next arrival
                 time.sleep(max(0.0, pending[i].arrival - now))
                 continue
-            # work is in flight: look again shortly (new arrivals, completions)
+            # the queue is full: look again shortly (new arrivals, completions)
             time.sleep(5e-5)
         torch.cuda.synchronize(self.device)
-        for st in self.req_streams + [self.tick_stream]:
+        for st in self.streams:
             main.wait_stream(st)
         self.makespan = time.perf_counter() - t0
         return requests
