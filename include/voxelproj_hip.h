@@ -167,7 +167,7 @@ VP_API int vp_vshift(int z_order, int transpose, const float *in, float *out, in
  * descriptor array twice: `items_host` (host memory, used for validation and
  * to size the launch) and `items_dev` (an identical copy in device memory,
  * read by the kernel).  `variant` is the number of H slices per lane of the
- * kernel configuration (forward: 1, 2 or 4; backward: 1 or 2); see
+ * kernel configuration (1, 2 or 4 for either operator); see
  * vp_default_variant.  `grid` is the number of persistent workgroups
  * (<= 0: one per work item).  Results do not depend on the grid or on the
  * other instances in the batch.  Because the launch reads only device
@@ -188,8 +188,10 @@ VP_API int vp_backward_batched(int variant, const vp_batch_item *items_host,
 
 /* Default kernel variant (slices per lane) of op 0 (A) / 1 (AT) for H slices,
  * the grouping key of batched launches.  vp_forward / vp_backward use it too,
- * except that 2 drops to 1 when the launch would have fewer workgroups than
- * the device has compute units (the two variants give identical results). */
+ * except that 2 may drop to 1 for small and mid-size launches (fewer
+ * workgroups than compute units, or fewer rounds of resident workgroups);
+ * variants 1 and 2 give identical results, 4 adds the same terms in another
+ * order (agreement to float32 rounding). */
 VP_API int vp_default_variant(int op, int64_t H);
 
 /* Number of work items (workgroups) one instance contributes to a batched

@@ -75,3 +75,24 @@ def test_fifo_plain_baseline():
     fifo = serve.run_plain_fifo(reqs)
     assert 0 <= fifo["p50_ms"] <= fifo["p99_ms"]
     assert fifo["makespan_s"] >= max(r.arrival for r in reqs)
+
+
+@pytest.mark.parametrize("n_iter", [0, 1])
+def test_sirt_with_zero_and_one_iteration_equals_plain(n_iter):
+    # zero iterations: the weights are computed and the result is the zero
+    # start volume, as recon.SIRT.iterate gives; one iteration: one update
+    reqs = [r for r in small_mix(30, seed=9) if r.kind == "SIRT"][:2]
+    for r in reqs:
+        r.n_iter = n_iter
+    serve.Server().run(reqs)
+    plain, _ = serve.run_plain(reqs)
+    for r, p in zip(reqs, plain):
+        assert torch.equal(r.result, p), (r.rid, n_iter)
+        if n_iter == 0:
+            assert not bool(r.result.any())
+
+
+def test_negative_iteration_count_is_rejected():
+    r = small_mix(30, seed=9)[0]
+    with pytest.raises(ValueError):
+        serve.Request(r.rid, "SIRT", 0.0, r.data, r.angles, r.SY, r.SX, n_iter=-1)
