@@ -173,4 +173,21 @@ __device__ __forceinline__ void dma_to_lds(const void* base, uint32_t range, uin
         __builtin_amdgcn_raw_ptr_buffer_load_lds(r, lds, 4, voff, 0, 0, 0);
 }
 
+// The same transfer from a descriptor built once for many instructions
+// (dma_rsrc), with a wave-uniform byte offset `soff` (an SGPR) added to
+// every lane's address: one scalar add per instruction instead of a new
+// descriptor.  The caller keeps soff + voff + BYTES within `range`, so no
+// lane depends on how the range check treats soff.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t dma_rsrc(const void* base, uint32_t range)
+{
+    constexpr int kRsrcWord3 = 0x00020000;  // raw 32-bit buffer, gfx9 family
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)range, kRsrcWord3);
+}
+
+__device__ __forceinline__ void dma16_to_lds(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff,
+                                             __attribute__((address_space(3))) void* lds)
+{
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, lds, 16, voff, soff, 0, 0);
+}
+
 }  // namespace vp

@@ -4,17 +4,20 @@
 // per visit.  Here A is evaluated as the exact transpose of the voxel-driven
 // footprint (the same per-voxel r and weights as the backprojector), gathered
 // into output-stationary accumulators:
-//   wave  = one angle x P consecutive detector pixels x 64 H slices (lane = slice);
+//   sweep wave = one angle x P consecutive detector pixels x 64*HL H slices
+//                (lane = HL consecutive slices);
 //   rows  = lines of the volume along the axis whose direction cosine is
 //           larger (|kappa| >= 1/sqrt 2), so consecutive voxels of a line
 //           are at least 1/sqrt 2 pixel apart on the detector and at most
 //           two of them share the first pixel l of their footprint;
-//   per line, lane m evaluates the footprint of the m-th voxel of the line
-//   segment that touches the wave's pixels (in increasing-u order), and a
-//   statically unrolled sweep over l = p0-2 .. p0+P-1 adds each voxel to
-//   accumulators l, l+1 and, for the one voxel-angle in eight whose
-//   footprint covers three pixels, l+2 -- so every accumulator index is a
-//   compile-time constant and only the voxel index is dynamic.
+//   per line, lane m of a staging wave evaluates the footprint of the m-th
+//   voxel of the line segment that touches an angle's pixels (in
+//   increasing-u order) and files it in a slot table by its first pixel l;
+//   the angle's sweep wave then runs a statically unrolled sweep over
+//   l = p0-2 .. p0+P-1 that adds each voxel to accumulators l, l+1 and, for
+//   the one voxel-angle in eight whose footprint covers three pixels, l+2 --
+//   so every accumulator index is a compile-time constant and only the voxel
+//   index is dynamic.
 // No atomics; the summation order is fixed, so results are deterministic.
 #include "vp_device.hpp"
 #include "vp_launch.hpp"
@@ -43,15 +46,6 @@ struct Task {
     int row, j0, j1;
 };
 
-// Footprint slot table of one piece, lane = slot q: t0 a voxel whose
-// footprint starts at pixel p0 - 2 + q, as (wl, wr, w3, LDS offset of the
-// voxel's staged slices), and t1 the other one (if two do), as (wl, wr, w3,
-// 0).  Of two such voxels (neighbours on the line), t0 is the one staged
-// first and t1 the next staged voxel, so t1's offset is t0's plus one voxel.
-struct Slots {
-    float4 t0, t1;
-};
-
 // Value of lane - 1 (lane 0: 0), by a DPP wavefront shift.
 __device__ __forceinline__ int lane_shift_up(int v)
 {
@@ -66,39 +60,22 @@ __device__ __forceinline__ int lane_shift_down(int v)
     return __builtin_amdgcn_update_dpp(0, v, kWaveShl1, 0xf, 0xf, false);
 }
 
-// Cross-lane scatter of (a, b, c, o): lane d/4 of the result receives the
-// values of the highest-numbered lane whose byte address d points at it;
-// lanes that no lane points at receive zeros (ds_permute_b32 semantics).
-__device__ __forceinline__ float4 permute4(int d, float a, float b, float c, int o)
-{
-    float4 r;
-    r.x = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(d, __builtin_bit_cast(int, a)));
-    r.y = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(d, __builtin_bit_cast(int, b)));
-    r.z = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(d, __builtin_bit_cast(int, c)));
-    r.w = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(d, o));
-    return r;
-}
-
-// The same for (a, b, c) only; the fourth component is zero.
-__device__ __forceinline__ float4 permute3(int d, float a, float b, float c)
-{
-    float4 r;
-    r.x = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(d, __builtin_bit_cast(int, a)));
-    r.y = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(d, __builtin_bit_cast(int, b)));
-    r.z = __builtin_bit_cast(float, __builtin_amdgcn_ds_permute(d, __builtin_bit_cast(int, c)));
-    r.w = 0.0f;
-    return r;
-}
-
-// Block = NW angles (one per wave) x P detector pixels x 64*HL slices.
-// For every line of the volume along the angles' dominant axis, the union of
-// the NW line segments that touch the block's pixels is staged in LDS once
-// (HL slices per lane, 256*HL bytes per voxel) and read by all NW waves, so
-// each volume load feeds up to NW angles.  Lines are split into pieces of at
-// most UMAX voxels; while one piece is swept, the next streams into the other
-// LDS buffer by DMA and each wave builds its footprint slot table for it (one
-// barrier per piece).  Angles whose dominant axis is x and those whose
-// dominant axis is y are handled in two passes over the volume.
+// Block = NW sweep waves (one angle each) x P detector pixels x 64*HL slices,
+// plus NS staging waves.  For every line of the volume along the angles'
+// dominant axis, the union of the NW line segments that touch the block's
+// pixels is staged in LDS once (HL slices per lane, 256*HL bytes per voxel)
+// and read by all NW sweep waves, so each volume load feeds up to NW angles.
+// Lines are split into pieces of at most UMAX voxels.  The work of a piece is
+// split by role, one barrier per piece:
+//   staging waves: find the next piece, issue its global->LDS DMA into the
+//     other buffer, and build the footprint slot table of each of their AS
+//     angles for it in LDS;
+//   sweep waves: read their angle's slot table for the current piece and
+//     sweep it (the FMAs).
+// So the latency-bound part of a piece (piece iteration, footprints, slot
+// tables, the DMA in flight) overlaps the sweep of the previous piece instead
+// of preceding it in every wave.  Angles whose dominant axis is x and those
+// whose dominant axis is y are handled in two passes over the volume.
 template <int P, int NW, int HL>
 __host__ __device__ inline void forward_grid(int A, int64_t H, int64_t PX, int& nbp, int& nba, int& nbh)
 {
@@ -107,7 +84,15 @@ __host__ __device__ inline void forward_grid(int A, int64_t H, int64_t PX, int& 
     nbh = (int)((H + 64 * HL - 1) / (64 * HL));
 }
 
-template <int P, int NW, int HL, int UMAX>
+// A block of NW sweep waves and NS staging waves.
+template <int NW, int NS>
+struct FwdRoles {
+    static_assert(NW + NS <= 16, "at most 1024 threads per block");
+    static constexpr int AS = (NW + NS - 1) / NS;  // angles per staging wave
+    static constexpr int THREADS = 64 * (NW + NS);
+};
+
+template <int P, int NW, int NS, int HL, int UMAX>
 __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical)
 {
     const float* __restrict__ x = g.in;
@@ -119,6 +104,9 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     int nbp, nba, nbh;
     forward_grid<P, NW, HL>(A, H, PX, nbp, nba, nbh);
     static_assert((P + 5) * 1.4143 + 4 <= kSeg, "segment may exceed one wave");
+    constexpr int AS = FwdRoles<NW, NS>::AS;
+    constexpr int SLOTS = P + 2;                 // first pixels p0-2 .. p0+P-1
+    static_assert(SLOTS <= 32, "slot masks are 32-bit");
     constexpr int VOX = 64 * HL;                 // floats per staged voxel
     // floats per buffer: UMAX voxels plus one, so that the sweep's read of a
     // slot's second voxel (one voxel above the first) stays inside the array
@@ -130,8 +118,15 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     // the DMA before them.
     __shared__ float4 s_xa[STAGE / 4];
     __shared__ float4 s_xb[STAGE / 4];
+    // Slot tables [buffer][angle][rank][slot]: slot q of rank r holds the r-th
+    // voxel whose footprint starts at pixel p0 - 2 + q as (wl, wr, w3, LDS
+    // offset of its staged slices); empty slots hold zeros.
+    __shared__ float4 s_tab[2][NW][2][SLOTS];
+    // piece k in s_rec[k % 2]: (line, first voxel, last voxel, pass); line -1 after the last
+    __shared__ int4 s_rec[2];
     __shared__ AngleK s_k[NW];
     __shared__ int s_mode[NW];
+    __shared__ float4 s_trash[64];  // discarded table writes, one slot per lane
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -141,7 +136,6 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     const int bp = (logical / nba) % nbp;
     const int bh = logical / (nba * nbp);
 
-    const int a = ba * NW + wave;
     const int hb = bh * VOX;                     // first slice of the block
     const int p0 = bp * P;
     const int pl = p0 - 1;
@@ -159,323 +153,150 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
         s_mode[tid] = (at < A) ? (fabsf(cc) >= fabsf(ss) ? 0 : 1) : -1;
     }
     __syncthreads();
-    AngleK k = s_k[wave];
-    k.chi = uni(k.chi); k.clo = uni(k.clo); k.shi = uni(k.shi); k.slo = uni(k.slo);
-    k.a = uni(k.a); k.b = uni(k.b); k.h = uni(k.h); k.cd = uni(k.cd);
-    k.pch = uni(k.pch); k.plo = uni(k.plo);
-    const int mode = uni(s_mode[wave]);
-    // Lane w < NW describes angle w of the block.
-    const AngleK kl = s_k[lane < NW ? lane : 0];
-    const int model = (lane < NW) ? s_mode[lane] : -1;
 
-    // Register partial sums over at most kRowsPerFold lines; the owning wave
-    // folds them into y (it is the only writer of its outputs), which keeps the
-    // FP32 summation two-level without a second register array.
-    float part[P][HL];
+    // Staging DMA of a piece, issued by the staging waves: staging wave s
+    // issues the instructions s, s + NS, ...; one instruction moves RPI
+    // voxels of 64*HL slices (16 B per lane when H is a multiple of 4; else
+    // HL dword instructions per voxel).  Idle lanes past H load a clamped,
+    // valid address; their results are never stored.  Staged slot v holds
+    // voxel j0 + v (the same order for every angle of the block, whatever its
+    // direction along the line).
+    constexpr int RPI = 1024 / (VOX * 4);    // voxels per 16-byte-per-lane DMA
+    constexpr int LPR = 64 / RPI;            // lanes per voxel
+    static_assert(UMAX % RPI == 0, "staging buffer must hold whole DMA groups");
+    struct Dma {
+        const float* base;
+        int64_t st_along;
+        int nv, nins;
+        bool fast;
+        __amdgpu_buffer_rsrc_t r;
+    };
+    // (line, first voxel, last voxel, pass) of a piece
+    auto plan = [&](const int row, const int j0, const int j1, const int pass) {
+        Dma d;
+        const bool px = pass == 0;
+        d.st_along = px ? (int64_t)H : (int64_t)SX * H;
+        const int64_t st_row = px ? (int64_t)SX * H : (int64_t)H;
+        d.nv = j1 - j0 + 1;
+        d.base = x + (int64_t)row * st_row + (int64_t)j0 * d.st_along;
+        d.nins = vec4 ? (d.nv + RPI - 1) / RPI : d.nv;
+        // one descriptor for the piece when its extent fits 31 bits
+        const int64_t extent = ((int64_t)(d.nv - 1) * d.st_along + H) * 4;
+        d.fast = extent < (1ll << 31);
+        d.r = dma_rsrc(d.base, (uint32_t)(d.fast ? extent : 0));
+        return d;
+    };
+    auto issue = [&](const Dma& d, float4* dst, const int ins) __attribute__((always_inline)) {
+        if (ins >= d.nins) return;
+        if (vec4) {
+            const int v0 = ins * RPI;
+            // lanes past the piece (last instruction, RPI > 1) load its last voxel again
+            const int v = min(v0 + lane / LPR, d.nv - 1);
+            const int hoff = min(hb + (lane % LPR) * 4, H - 4);
+            __attribute__((address_space(3))) void* lds =
+                (__attribute__((address_space(3))) void*)&dst[v0 * (VOX / 4)];
+            if (d.fast) {
+                dma16_to_lds(d.r, (uint32_t)(((int64_t)(v - v0) * d.st_along + hoff) * 4),
+                             (uint32_t)((int64_t)v0 * d.st_along * 4), lds);
+            } else {
+                // lane offsets below (RPI-1) * st_along + H floats, under 2^29
+                // (RPI > 1 only for up to 128 slices in single launches;
+                // batched launches check it)
+                dma_to_lds<16>(d.base + (int64_t)v0 * d.st_along,
+                               (uint32_t)(((int64_t)(RPI - 1) * d.st_along + H) * 4),
+                               (uint32_t)(((int64_t)(v - v0) * d.st_along + hoff) * 4), lds);
+            }
+        } else {
 #pragma unroll
-    for (int i = 0; i < P; ++i)
-#pragma unroll
-        for (int j = 0; j < HL; ++j) part[i][j] = 0.0f;
-    bool folded = false;
-    auto fold = [&]() {
-        if (a < A) {
-            const int hh0 = hb + lane * HL;
-            const bool full = vec4 && hh0 + 3 < H;
-#pragma unroll
-            for (int i = 0; i < P; ++i) {
-                const int p = p0 + i;
-                if (p < PX) {
-                    float* py = y + ((int64_t)a * PX + p) * H + hh0;
-                    bool done = false;
-                    if constexpr (HL == 4) {
-                        if (full) {
-                            // one 16-byte read-modify-write per lane: whole cache lines
-                            float4 v = make_float4(part[i][0], part[i][1], part[i][2], part[i][3]);
-                            if (accumulate || folded) {
-                                const float4 o = *reinterpret_cast<const float4*>(py);
-                                v.x += o.x; v.y += o.y; v.z += o.z; v.w += o.w;
-                            }
-                            *reinterpret_cast<float4*>(py) = v;
-                            done = true;
-                        }
-                    }
-                    if (!done) {
-#pragma unroll
-                        for (int j = 0; j < HL; ++j) {
-                            if (hh0 + j < H) py[j] = (accumulate || folded) ? (py[j] + part[i][j]) : part[i][j];
-                        }
-                    }
-                }
+            for (int c = 0; c < HL; ++c) {
+                // dword instruction c fills floats [64c, 64c+64) of the
+                // voxel's row, i.e. slices hb + 64c + lane.
+                const int hh = min(hb + 64 * c + lane, H - 1);
+                dma_to_lds<4>(d.base + (int64_t)ins * d.st_along, (uint32_t)H * 4, (uint32_t)hh * 4,
+                              (__attribute__((address_space(3))) void*)(reinterpret_cast<float*>(&dst[ins * (VOX / 4)]) + 64 * c));
             }
         }
+    };
+
+    if (wave < NW) {
+        // ---------------------------------------------------------------- sweep
+        const int a = ba * NW + wave;
+        // Register partial sums over at most kRowsPerFold lines of one pass;
+        // the owning wave folds them into y (it is the only writer of its
+        // outputs), which keeps the FP32 summation two-level without a second
+        // register array.
+        float part[P][HL];
 #pragma unroll
         for (int i = 0; i < P; ++i)
 #pragma unroll
             for (int j = 0; j < HL; ++j) part[i][j] = 0.0f;
-        folded = true;
-        // Retire the fold's loads and stores here.  Left in flight, they make
-        // the compiler wait on vmcnt before the next sweep's LDS reads, which
-        // also waits for the next piece's DMA issued meanwhile (about 1 % of
-        // A at 512^3 x 192).
-        __builtin_amdgcn_s_waitcnt(0);
-    };
-
-    for (int pass = 0; pass < 2; ++pass) {
-        const bool any = __ballot(model == pass) != 0ull;
-        if (!any) continue;  // block-uniform: every wave sees the same modes
-        const bool mine = (mode == pass);
-        const bool px_ = (pass == 0);
-        const int nrows = px_ ? SY : SX;
-        const int nalong = px_ ? SX : SY;
-        const float ca = px_ ? xc : yc;
-        const float cr = px_ ? yc : xc;
-        const int64_t st_along = px_ ? (int64_t)H : (int64_t)SX * H;
-        const int64_t st_row = px_ ? (int64_t)SX * H : (int64_t)H;
-        const float kap = px_ ? (k.chi + k.clo) : (k.shi + k.slo);
-        const float oth = px_ ? (k.shi + k.slo) : (k.chi + k.clo);
-        const float ikap = 1.0f / kap;
-        // Direction in which the centre pixel grows along a line.
-        const bool fwd = kap >= 0.0f;
-        // Lane w < NW: the (kap, 1/kap, oth) of angle w of the block, read
-        // with v_readlane where the union of the block's segments is formed.
-        const float kcl = kl.chi + kl.clo, ksl = kl.shi + kl.slo;
-        const float kp_l = px_ ? kcl : ksl;
-        const float ot_l = px_ ? ksl : kcl;
-        const float ik_l = 1.0f / kp_l;
-        const float pc_l = kl.pch + kl.plo;  // pc - du of angle w (segment bounds only)
-
-        // Segment [jlo, jhi] of line R (centred row coordinate) for an angle
-        // (kap, oth): voxels whose centre pixel can lie in [pl, ph], with one
-        // pixel of margin each side.
-        auto segment = [&](float R, float kp, float ikp, float ot, float pcw, int& jlo, int& jhi) {
-            const float ub = fmaf(-ca, kp, fmaf(R, ot, pcw));
-            const float ja = ((float)pl - 1.5f - ub) * ikp;
-            const float jb = ((float)ph + 1.5f - ub) * ikp;
-            jlo = max(0, (int)floorf(fminf(ja, jb)) - 1);
-            jhi = min(nalong - 1, (int)ceilf(fmaxf(ja, jb)) + 1);
-        };
-        // Line table for 64 consecutive lines c0 .. c0+63, lane = line: the
-        // union of the block's segments (what is staged) and this wave's own
-        // segment; `nonempty` marks lines with anything to stage.
-        int c0 = -64;
-        int u_lo = 0, u_hi = -1, s_lo = 0, s_hi = -1;
-        uint64_t nonempty = 0;
-        auto load_lines = [&](int base) {
-            c0 = base;
-            const int row = base + lane;
-            const float R = (float)row - cr;
-            int lo = 0x7fffffff, hi = -1;
-            // not unrolled: keeps the per-angle reads inside the loop (unrolled,
-            // they are hoisted to every caller of the line search)
-#pragma unroll 1
-            for (int w = 0; w < NW; ++w) {
-                if (__builtin_amdgcn_readlane(model, w) == pass) {
-                    int l, h;
-                    segment(R, rdlane(kp_l, w), rdlane(ik_l, w), rdlane(ot_l, w), rdlane(pc_l, w), l, h);
-                    if (h >= l) {
-                        lo = min(lo, l);
-                        hi = max(hi, h);
-                    }
-                }
-            }
-            const bool valid = row < nrows;
-            u_lo = lo;
-            u_hi = valid ? hi : -1;
-            nonempty = __ballot(valid && hi >= lo);
-            s_lo = 0;
-            s_hi = -1;
-            if (mine) segment(R, kap, ikap, oth, k.pch + k.plo, s_lo, s_hi);
-        };
-        // First piece at or after line `row` (pieces hold at most UMAX voxels).
-        auto first_task = [&](int row) {
-            Task t{-1, 0, -1};
-            while (row < nrows) {
-                if (row >= c0 + 64) load_lines(row);
-                const uint64_t m = nonempty & (~0ull << (row - c0));
-                if (m) {
-                    row = c0 + (int)__builtin_ctzll(m);
-                    const int ulo = uni(__builtin_amdgcn_readlane(u_lo, row - c0));
-                    const int uhi = uni(__builtin_amdgcn_readlane(u_hi, row - c0));
-                    t = Task{row, ulo, min(uhi, ulo + UMAX - 1)};
-                    break;
-                }
-                row = c0 + 64;
-            }
-            return t;
-        };
-        // The line of `t` is always in the current table (lines only advance).
-        auto next_task = [&](const Task& t) {
-            const int uhi = uni(__builtin_amdgcn_readlane(u_hi, t.row - c0));
-            if (t.j1 < uhi) return Task{t.row, t.j1 + 1, min(uhi, t.j1 + UMAX)};
-            return first_task(t.row + 1);
-        };
-        // Stage a piece straight into LDS with global->LDS DMA: one wave
-        // instruction moves one voxel's 64*HL slices (16 B per lane when H is a
-        // multiple of 4, else HL dword instructions).  Idle lanes past H load a
-        // clamped, valid address; their results are never stored.  Staged
-        // slot s holds voxel j0 + s (the order is the same for every wave of
-        // the block, whatever the direction of its angle).
-        auto fetch = [&](const Task& t, float4* dst) __attribute__((always_inline)) {
-            const int nv = t.j1 - t.j0 + 1;
-            const float* base = x + (int64_t)t.row * st_row + (int64_t)t.j0 * st_along;
-            // Buffer-form DMA: each instruction's base is its first voxel;
-            // lane offsets stay below (RPI-1) * st_along + H floats, under
-            // 2^29 (RPI > 1 only for up to 128 slices in single launches;
-            // batched launches check it).
-            if (vec4) {
-                constexpr int RPI = 1024 / (VOX * 4);  // voxels per 16-byte-per-lane DMA
-                constexpr int LPR = 64 / RPI;          // lanes per voxel
-                static_assert(UMAX % RPI == 0, "staging buffer must hold whole DMA groups");
-                const int sub = lane / LPR;
-                const int hoff = min(hb + (lane % LPR) * 4, H - 4);
-                const int nins = (nv + RPI - 1) / RPI;
-                const int nfull = nv / RPI;  // instructions whose RPI voxels all exist
-                const uint32_t range = (uint32_t)(((int64_t)(RPI - 1) * st_along + H) * 4);
-                // full instructions: the same lane offsets every time, the base
-                // and the LDS destination advance by constant steps
-                const uint32_t voff = (uint32_t)(((int64_t)sub * st_along + hoff) * 4);
-                const int64_t step = (int64_t)NW * RPI * st_along;
-                const float* src = base + (int64_t)wave * RPI * st_along;
-                __attribute__((address_space(3))) float4* d =
-                    (__attribute__((address_space(3))) float4*)&dst[wave * RPI * (VOX / 4)];
-                int ins = wave;
-                for (; ins < nfull; ins += NW) {
-                    dma_to_lds<16>(src, range, voff, (__attribute__((address_space(3))) void*)d);
-                    src += step;
-                    d += NW * RPI * (VOX / 4);
-                }
-                if (ins < nins) {
-                    // the last, partial instruction (RPI > 1 only): lanes past
-                    // the piece load its last voxel again
-                    const int v0 = ins * RPI;
-                    const int v = min(v0 + sub, nv - 1);
-                    dma_to_lds<16>(base + (int64_t)v0 * st_along, range,
-                                   (uint32_t)(((int64_t)(v - v0) * st_along + hoff) * 4),
-                                   (__attribute__((address_space(3))) void*)&dst[ins * RPI * (VOX / 4)]);
-                }
-            } else {
-                for (int v = wave; v < nv; v += NW) {
+        bool folded = false;
+        auto fold = [&]() {
+            if (a < A) {
+                const int hh0 = hb + lane * HL;
+                const bool full = vec4 && hh0 + 3 < H;
 #pragma unroll
-                    for (int c = 0; c < HL; ++c) {
-                        // dword instruction c fills floats [64c, 64c+64) of the
-                        // voxel's row, i.e. slices hb + 64c + lane.
-                        const int e = 64 * c + lane;           // float index in the row
-                        const int hh = min(hb + e, H - 1);
-                        dma_to_lds<4>(base + (int64_t)v * st_along, (uint32_t)H * 4, (uint32_t)hh * 4,
-                                      (__attribute__((address_space(3))) void*)(reinterpret_cast<float*>(&dst[v * (VOX / 4)]) + 64 * c));
+                for (int i = 0; i < P; ++i) {
+                    const int p = p0 + i;
+                    if (p < PX) {
+                        float* py = y + ((int64_t)a * PX + p) * H + hh0;
+                        bool done = false;
+                        if constexpr (HL == 4) {
+                            if (full) {
+                                // one 16-byte read-modify-write per lane: whole cache lines
+                                float4 v = make_float4(part[i][0], part[i][1], part[i][2], part[i][3]);
+                                if (accumulate || folded) {
+                                    const float4 o = *reinterpret_cast<const float4*>(py);
+                                    v.x += o.x; v.y += o.y; v.z += o.z; v.w += o.w;
+                                }
+                                *reinterpret_cast<float4*>(py) = v;
+                                done = true;
+                            }
+                        }
+                        if (!done) {
+#pragma unroll
+                            for (int j = 0; j < HL; ++j) {
+                                if (hh0 + j < H) py[j] = (accumulate || folded) ? (py[j] + part[i][j]) : part[i][j];
+                            }
+                        }
                     }
                 }
             }
+#pragma unroll
+            for (int i = 0; i < P; ++i)
+#pragma unroll
+                for (int j = 0; j < HL; ++j) part[i][j] = 0.0f;
+            folded = true;
         };
-
-        // Footprints of this wave's voxels in piece t (lane = voxel), moved
-        // into the slot table st (lane = slot) by cross-lane permutes: slot
-        // (rank, q) holds the rank-th voxel whose footprint starts at pixel
-        // p0 - 2 + q (at most two per pixel) as (wl, wr, w3, LDS offset).
-        // Empty slots hold zero weights and offset 0, so the sweep is
-        // straight-line code.  Returns whether the wave has anything to add
-        // from this piece.
-        auto setup = [&](const Task& t, Slots& st) -> int {
-            if (!mine) return 0;
-            const int jlo = uni(__builtin_amdgcn_readlane(s_lo, t.row - c0));
-            const int jhi = uni(__builtin_amdgcn_readlane(s_hi, t.row - c0));
-            const int n = jhi - jlo + 1;
-            if (n <= 0) return 0;
-            const float R = (float)t.row - cr;
-            const int mcl = min(lane, n - 1);
-            const int jm = fwd ? jlo + mcl : jhi - mcl;
-            // footprint anchored at its first nonzero pixel l; slot q = l - (pl - 1)
-            // covers l in [p0-2, p0+P-1], the voxels that touch the block's pixels
-            const Taps2 tp = anchor_left(
-                voxel_taps(px_ ? (float)jm - xc : R, px_ ? R : (float)jm - yc, k));
-            const int lprev = lane_shift_up(tp.l);    // lane m-1 holds the previous voxel
-            const int lnext = lane_shift_down(tp.l);  // lane m+1 the next
-            const bool valid = lane < n;
-            const int l = valid ? tp.l : 0x3fffffff;
-            const int mm0 = uni((int)__popcll(__ballot(valid && l < pl - 1)));
-            const int mend = uni((int)__popcll(__ballot(valid && l <= ph - 1)));
-            int mA = fwd ? t.j0 - jlo : jhi - t.j1;
-            int mB = fwd ? t.j1 - jlo : jhi - t.j0;
-            mA = uni(max(mA, mm0));
-            mB = uni(min(mB, mend - 1));
-            if (mA > mB) return 0;
-            const int jb0 = fwd ? (jlo - t.j0) : (jhi - t.j0);  // staged slot of lane 0's voxel
-            const int dj = fwd ? 1 : -1;
-            // Destination lane of each voxel in the rank-0 and rank-1 tables;
-            // voxels outside [mA, mB] go to lane 63, which is never a slot.
-            // Of two voxels with the same l (lanes m-1, m), rank 1 is the one
-            // staged later: lane m when j grows with the lane (fwd), else
-            // lane m-1.
-            const bool in = lane >= mA && lane <= mB;
-            const bool second = fwd ? (lane > mA && l == lprev) : (lane < mB && l == lnext);
-            const int q = l - (pl - 1);
-            const int d0 = 4 * ((in && !second) ? q : 63);
-            const int d1 = 4 * ((in && second) ? q : 63);
-            const int off = (jb0 + dj * lane) * VOX;
-            st.t0 = permute4(d0, tp.wl, tp.wr, tp.w3, off);
-            st.t1 = permute3(d1, tp.wl, tp.wr, tp.w3);  // its offset is t0's plus one voxel
-            return 1;
-        };
-
-        // Pipeline over pieces: while one piece is swept from one buffer, the
-        // next streams into the other buffer and its slot table is built.
-        __syncthreads();  // previous pass's readers of s_xa / s_xb are done
-        Task cur = first_task(0);
-        int flag = 0;
-        Slots sl;
-        if (cur.row >= 0) {
-            fetch(cur, s_xa);
-            flag = setup(cur, sl);
-        }
-        __builtin_amdgcn_s_waitcnt(0);
-        __syncthreads();
-        int rows_done = 0;
-        // One piece: sweep `cur` from xcur with slot table sl; stage the next
-        // piece into xnxt and build its table.  Inlined at two call sites so
-        // that both buffers are compile-time known.
-        auto step = [&](const float4* xcur, float4* xnxt) __attribute__((always_inline)) {
-            const Task nxt = next_task(cur);
-            int flag_n = 0;
-            Slots sl_n;
-            // Staging DMA of the next piece.  With one voxel per instruction
-            // (4 slices per lane), each wave issues its instruction k at pixel
-            // slot 2k of its sweep instead of all of them at once right after
-            // the barrier, when every wave of the block queues its DMAs at the
-            // same time (1-4 % faster; same data, same LDS slots).
-            constexpr bool kSpread = (VOX * 4 == 1024);
-            constexpr int kMaxIssue = (UMAX + NW - 1) / NW;
-            constexpr int kInSweep = kMaxIssue < (P + 3) / 2 ? kMaxIssue : (P + 3) / 2;
-            int dcnt = 0;
-            const float* dsrc = x;
-            float4* dd = xnxt;
-            int64_t dstep = 0;
-            uint32_t dvoff = 0, drange = 0;
-            if (nxt.row >= 0) {
-                flag_n = setup(nxt, sl_n);
-                if (kSpread && vec4) {
-                    const int nv = nxt.j1 - nxt.j0 + 1;
-                    dcnt = wave < nv ? (nv - wave + NW - 1) / NW : 0;
-                    dsrc = x + (int64_t)nxt.row * st_row + (int64_t)(nxt.j0 + wave) * st_along;
-                    dd = xnxt + wave * (VOX / 4);
-                    dstep = (int64_t)NW * st_along;
-                    dvoff = (uint32_t)min(hb + lane * 4, H - 4) * 4;
-                    drange = (uint32_t)H * 4;
-                } else {
-                    fetch(nxt, xnxt);
-                }
+        int fpass = 0, frow = 0;  // fold group (pass, line group) of the partial sums
+        // One piece: sweep the piece in buffer b (xcur) with its slot table.
+        // Returns false at the end marker.  Inlined at two call sites so that
+        // both buffers are compile-time known.
+        auto step = [&](const float4* xcur, const int b) __attribute__((always_inline)) -> bool {
+            // every LDS read of the piece's header first, so that their
+            // latencies overlap: the piece's record and its slot table
+            // (unmasked reads; lanes past the slots drop theirs)
+            const int4 rc = s_rec[b];
+            const int ql = lane < SLOTS ? lane : SLOTS - 1;
+            const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+            float4 t0 = s_tab[b][wave][0][ql];
+            float4 t1 = s_tab[b][wave][1][ql];
+            if (lane >= SLOTS) {
+                t0 = zero;
+                t1 = zero;
             }
-            auto issue = [&](int k) __attribute__((always_inline)) {
-                if (k < dcnt)
-                    dma_to_lds<16>(dsrc + k * dstep, drange, dvoff,
-                                   (__attribute__((address_space(3))) void*)(dd + k * NW * (VOX / 4)));
-            };
-            if (flag) {
-                const float4 t0 = sl.t0;
-                const float4 t1 = sl.t1;
-                // Slot masks, 32-bit (one scalar bit test each).  Only slot
-                // lanes 0 .. P+1 and the discard lane 63 receive table entries
-                // (the others read zero), and the cast drops lane 63.
-                static_assert(P + 2 <= 32, "slot masks are 32-bit");
+            // the angle adds something from the piece iff a slot holds a voxel
+            // (a voxel's first weight wl is positive; empty tables are zeros)
+            const bool swept = __ballot(t0.x != 0.f) != 0ull;
+            const int row = uni(rc.x);
+            if (row < 0) return false;
+            const int pass = uni(rc.w);
+            if (pass != fpass || row / kRowsPerFold != frow / kRowsPerFold) {
+                fold();
+                fpass = pass;
+                frow = row;
+            }
+            if (swept) {
                 using Mask = uint32_t;
                 // bit q: slot q holds a second voxel (about 1 in 10 at 512^3 x
                 // 192); a voxel's first weight wl is always positive
@@ -511,27 +332,26 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                 const float* src = slot_src(0);
                 auto xa_n = ld(src);
                 auto xb_n = ld(src + VOX);
-                // the rank-0 weights of pixel q+1 are read out one pixel ahead too
-                float wl_n = rdlane(t0.x, 0), wr_n = rdlane(t0.y, 0);
+                // The rank-0 weights (wl, wr) of pixel q+1 are read one pixel
+                // ahead, as a wave-uniform LDS read: VGPR operands, for the
+                // faster all-VGPR form of the FMAs and no v_readlane.
+                const float2* wtab = reinterpret_cast<const float2*>(&s_tab[b][wave][0][0]);
+                float2 wv_n = wtab[0];
 #pragma unroll
-                for (int q = 0; q < P + 2; ++q) {
-                    if (q % 2 == 0 && q / 2 < kInSweep) issue(q / 2);
+                for (int q = 0; q < SLOTS; ++q) {
                     const auto xa = xa_n;
                     const auto xb = xb_n;
-                    const float wl = wl_n, wr = wr_n;
-                    if (q + 1 < P + 2) {
+                    const float wl = wv_n.x, wr = wv_n.y;
+                    if (q + 1 < SLOTS) {
                         src = slot_src(q + 1);
                         xa_n = ld(src);
                         xb_n = ld(src + VOX);
-                        wl_n = rdlane(t0.x, q + 1);
-                        wr_n = rdlane(t0.y, q + 1);
+                        wv_n = wtab[2 * (q + 1)];  // float4 entries: (wl, wr) at float2 index 2q
                     }
-                    {
 #pragma unroll
-                        for (int j = 0; j < HL; ++j) {
-                            if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wl, xa.a[j], part[q - 2][j]);
-                            if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(wr, xa.a[j], part[q - 1][j]);
-                        }
+                    for (int j = 0; j < HL; ++j) {
+                        if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wl, xa.a[j], part[q - 2][j]);
+                        if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(wr, xa.a[j], part[q - 1][j]);
                     }
                     // one test for the common case of a slot with neither a
                     // three-pixel rank-0 footprint nor a second voxel
@@ -542,12 +362,12 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                             for (int j = 0; j < HL; ++j) part[q][j] = fmaf(w3, xa.a[j], part[q][j]);
                         }
                         if ((has2 >> q) & 1u) {
-                            const float wl = rdlane(t1.x, q);
-                            const float wr = rdlane(t1.y, q);
+                            const float wl2 = rdlane(t1.x, q);
+                            const float wr2 = rdlane(t1.y, q);
 #pragma unroll
                             for (int j = 0; j < HL; ++j) {
-                                if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wl, xb.a[j], part[q - 2][j]);
-                                if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(wr, xb.a[j], part[q - 1][j]);
+                                if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wl2, xb.a[j], part[q - 2][j]);
+                                if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(wr2, xb.a[j], part[q - 1][j]);
                             }
                             if (q < P && ((has3b >> q) & 1u)) {
                                 const float w3 = rdlane(t1.z, q);
@@ -558,29 +378,242 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                     }
                 }
             }
-            for (int k = flag ? kInSweep : 0; k < dcnt; ++k) issue(k);  // not issued in a sweep
-            __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA into xnxt has landed
-            __syncthreads();
-            cur = nxt;
-            flag = flag_n;
-            sl = sl_n;
+            __syncthreads();  // the piece's buffer, table and record may be reused
+            return true;
         };
-        while (cur.row >= 0) {
-            // Fold the partial sums when a new group of kRowsPerFold lines
-            // starts (checked once per pair of pieces, so the fold code exists
-            // once; its global loads retire before the next DMA is issued, as
-            // vmcnt retires in order).
-            if (cur.row / kRowsPerFold != rows_done / kRowsPerFold) {
-                fold();
-                rows_done = cur.row;
-            }
-            step(s_xa, s_xb);
-            if (cur.row < 0) break;
-            step(s_xb, s_xa);
+        __syncthreads();  // the first piece is staged
+        while (step(s_xa, 0) && step(s_xb, 1)) {
         }
         fold();
+        return;
     }
-    fold();
+
+    // ---------------------------------------------------------------- staging
+    // Staging waves issue first: their work is short latency-bound chains
+    // that the sweep waves' FMAs would otherwise delay, holding up the barrier.
+    __builtin_amdgcn_s_setprio(1);
+    const int s = wave - NW;  // staging wave s builds the tables of angles s*AS .. s*AS+AS-1
+    // Lane w < NW describes angle w of the block.
+    const AngleK kl = s_k[lane < NW ? lane : 0];
+    const int model = (lane < NW) ? s_mode[lane] : -1;
+    AngleK ka[AS];
+    int mode_a[AS];
+#pragma unroll
+    for (int i = 0; i < AS; ++i) {
+        const int w = s * AS + i;
+        const AngleK kk = s_k[w < NW ? w : 0];
+        ka[i].chi = uni(kk.chi); ka[i].clo = uni(kk.clo); ka[i].shi = uni(kk.shi); ka[i].slo = uni(kk.slo);
+        ka[i].a = uni(kk.a); ka[i].b = uni(kk.b); ka[i].h = uni(kk.h); ka[i].cd = uni(kk.cd);
+        ka[i].pch = uni(kk.pch); ka[i].plo = uni(kk.plo);
+        mode_a[i] = uni(w < NW ? s_mode[w] : -1);
+    }
+
+    // Pass state: lines run along x in pass 0 and along y in pass 1.
+    int pass = 0;
+    bool px_ = true;
+    int nrows = SY, nalong = SX;
+    float ca = xc, cr = yc;
+    int64_t st_along = H, st_row = (int64_t)SX * H;
+    float kp_l = 0.f, ik_l = 0.f, ot_l = 0.f, pc_l = 0.f;  // lane w < NW: angle w
+    float kap[AS], ikap[AS], oth[AS];
+    bool fwd[AS], mine[AS];
+    auto enter_pass = [&](int p) {
+        pass = p;
+        px_ = (p == 0);
+        nrows = px_ ? SY : SX;
+        nalong = px_ ? SX : SY;
+        ca = px_ ? xc : yc;
+        cr = px_ ? yc : xc;
+        st_along = px_ ? (int64_t)H : (int64_t)SX * H;
+        st_row = px_ ? (int64_t)SX * H : (int64_t)H;
+        const float kcl = kl.chi + kl.clo, ksl = kl.shi + kl.slo;
+        kp_l = px_ ? kcl : ksl;
+        ot_l = px_ ? ksl : kcl;
+        ik_l = 1.0f / kp_l;
+        pc_l = kl.pch + kl.plo;  // pc - du of angle w (segment bounds only)
+#pragma unroll
+        for (int i = 0; i < AS; ++i) {
+            kap[i] = px_ ? (ka[i].chi + ka[i].clo) : (ka[i].shi + ka[i].slo);
+            oth[i] = px_ ? (ka[i].shi + ka[i].slo) : (ka[i].chi + ka[i].clo);
+            ikap[i] = 1.0f / kap[i];
+            fwd[i] = kap[i] >= 0.0f;  // the centre pixel grows along the line
+            mine[i] = (mode_a[i] == p);
+        }
+    };
+
+    // Segment [jlo, jhi] of line R (centred row coordinate) for an angle
+    // (kap, oth): voxels whose centre pixel can lie in [pl, ph], with one
+    // pixel of margin each side.
+    auto segment = [&](float R, float kp, float ikp, float ot, float pcw, int& jlo, int& jhi) {
+        const float ub = fmaf(-ca, kp, fmaf(R, ot, pcw));
+        const float ja = ((float)pl - 1.5f - ub) * ikp;
+        const float jb = ((float)ph + 1.5f - ub) * ikp;
+        jlo = max(0, (int)floorf(fminf(ja, jb)) - 1);
+        jhi = min(nalong - 1, (int)ceilf(fmaxf(ja, jb)) + 1);
+    };
+    // Line table for 64 consecutive lines c0 .. c0+63, lane = line: the union
+    // of the block's segments (what is staged) and the segments of this
+    // wave's angles; `nonempty` marks lines with anything to stage.
+    int c0 = -64;
+    int u_lo = 0, u_hi = -1;
+    int s_lo[AS], s_hi[AS];
+#pragma unroll
+    for (int i = 0; i < AS; ++i) { s_lo[i] = 0; s_hi[i] = -1; }
+    uint64_t nonempty = 0;
+    auto load_lines = [&](int base) {
+        c0 = base;
+        const int row = base + lane;
+        const float R = (float)row - cr;
+        int lo = 0x7fffffff, hi = -1;
+        // not unrolled: keeps the per-angle reads inside the loop
+#pragma unroll 1
+        for (int w = 0; w < NW; ++w) {
+            if (__builtin_amdgcn_readlane(model, w) == pass) {
+                int l, h;
+                segment(R, rdlane(kp_l, w), rdlane(ik_l, w), rdlane(ot_l, w), rdlane(pc_l, w), l, h);
+                if (h >= l) {
+                    lo = min(lo, l);
+                    hi = max(hi, h);
+                }
+            }
+        }
+        const bool valid = row < nrows;
+        u_lo = lo;
+        u_hi = valid ? hi : -1;
+        nonempty = __ballot(valid && hi >= lo);
+#pragma unroll
+        for (int i = 0; i < AS; ++i) {
+            s_lo[i] = 0;
+            s_hi[i] = -1;
+            if (mine[i]) segment(R, kap[i], ikap[i], oth[i], ka[i].pch + ka[i].plo, s_lo[i], s_hi[i]);
+        }
+    };
+    // First piece at or after line `row` of the current pass (pieces hold at
+    // most UMAX voxels).
+    auto first_task = [&](int row) {
+        Task t{-1, 0, -1};
+        while (row < nrows) {
+            if (row >= c0 + 64) load_lines(row);
+            const uint64_t m = nonempty & (~0ull << (row - c0));
+            if (m) {
+                row = c0 + (int)__builtin_ctzll(m);
+                const int ulo = uni(__builtin_amdgcn_readlane(u_lo, row - c0));
+                const int uhi = uni(__builtin_amdgcn_readlane(u_hi, row - c0));
+                t = Task{row, ulo, min(uhi, ulo + UMAX - 1)};
+                break;
+            }
+            row = c0 + 64;
+        }
+        return t;
+    };
+    // First piece of pass p or of a later pass (row -1 when there is none).
+    auto first_from_pass = [&](int p) {
+        for (; p < 2; ++p) {
+            if (__ballot(model == p) == 0ull) continue;  // no angle of the block in pass p
+            enter_pass(p);
+            c0 = -64;
+            const Task t = first_task(0);
+            if (t.row >= 0) return t;
+        }
+        pass = 2;
+        return Task{-1, 0, -1};
+    };
+    // The line of `t` is always in the current table (lines only advance).
+    auto next_piece = [&](const Task& t) {
+        const int uhi = uni(__builtin_amdgcn_readlane(u_hi, t.row - c0));
+        if (t.j1 < uhi) return Task{t.row, t.j1 + 1, min(uhi, t.j1 + UMAX)};
+        const Task n = first_task(t.row + 1);
+        if (n.row >= 0) return n;
+        return first_from_pass(pass + 1);
+    };
+
+    // Slot tables of this wave's angles w = s*AS + i for piece t, in buffer
+    // b: footprints of each angle's voxels in the piece (lane = voxel), each
+    // written to slot (rank, q) for the rank-th voxel whose footprint starts
+    // at pixel p0 - 2 + q (at most two per pixel) as (wl, wr, w3, LDS offset;
+    // rank 1 without offset: it is rank 0's plus one voxel).  Empty slots hold
+    // zeros, so the sweep is straight-line code, and an angle with nothing
+    // to add from the piece gets an all-zero table.  Branch-free, so that the
+    // AS angles' dependency chains interleave: stores that are not wanted go
+    // to the lane's own word of s_trash.
+    auto tables = [&](const Task& t, const int b, auto&& mid) __attribute__((always_inline)) {
+        const float R = (float)t.row - cr;
+        int flag[AS];
+        Taps2 tp[AS];
+        int jlo[AS], jhi[AS], n[AS];
+#pragma unroll
+        for (int i = 0; i < AS; ++i) {
+            jlo[i] = uni(__builtin_amdgcn_readlane(s_lo[i], t.row - c0));
+            jhi[i] = uni(__builtin_amdgcn_readlane(s_hi[i], t.row - c0));
+            n[i] = (mine[i] && s * AS + i < NW) ? jhi[i] - jlo[i] + 1 : 0;
+            const int mcl = max(min(lane, n[i] - 1), 0);
+            const int jm = fwd[i] ? jlo[i] + mcl : jhi[i] - mcl;
+            // footprint anchored at its first nonzero pixel l; slot q = l - (pl - 1)
+            // covers l in [p0-2, p0+P-1], the voxels that touch the block's pixels
+            tp[i] = anchor_left(voxel_taps(px_ ? (float)jm - xc : R, px_ ? R : (float)jm - yc, ka[i]));
+        }
+        mid();
+#pragma unroll
+        for (int i = 0; i < AS; ++i) {
+            const bool fw = fwd[i];
+            const int lprev = lane_shift_up(tp[i].l);    // lane m-1 holds the previous voxel
+            const int lnext = lane_shift_down(tp[i].l);  // lane m+1 the next
+            const bool valid = lane < n[i];
+            const int l = valid ? tp[i].l : 0x3fffffff;
+            const int mm0 = uni((int)__popcll(__ballot(valid && l < pl - 1)));
+            const int mend = uni((int)__popcll(__ballot(valid && l <= ph - 1)));
+            const int mA = max(fw ? t.j0 - jlo[i] : jhi[i] - t.j1, mm0);
+            const int mB = min(fw ? t.j1 - jlo[i] : jhi[i] - t.j0, mend - 1);
+            flag[i] = (n[i] > 0 && mA <= mB) ? 1 : 0;
+            const int jb0 = fw ? (jlo[i] - t.j0) : (jhi[i] - t.j0);  // staged slot of lane 0's voxel
+            const int dj = fw ? 1 : -1;
+            // Of two voxels with the same l (lanes m-1, m), rank 1 is the one
+            // staged later: lane m when j grows with the lane (fw), else lane m-1.
+            const bool in = flag[i] && lane >= mA && lane <= mB;
+            const bool second = fw ? (lane > mA && l == lprev) : (lane < mB && l == lnext);
+            const int q = l - (pl - 1);
+            const int off = (jb0 + dj * lane) * VOX;
+            const int w = min(s * AS + i, NW - 1);
+            // the whole table is zeroed also when the angle has nothing to add
+            const bool zf = s * AS + i < NW && lane < SLOTS;
+            float4* z0 = zf ? &s_tab[b][w][0][lane] : &s_trash[lane];
+            float4* z1 = zf ? &s_tab[b][w][1][lane] : &s_trash[lane];
+            float4* dst = in ? &s_tab[b][w][second ? 1 : 0][in ? q : 0] : &s_trash[lane];
+            const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+            *z0 = zero;
+            *z1 = zero;
+            // after the zero fill: one wave's LDS writes land in order
+            *dst = make_float4(tp[i].wl, tp[i].wr, tp[i].w3, second ? 0.f : __builtin_bit_cast(float, off));
+        }
+    };
+    // Publish piece t (or the end marker) in buffer b: its DMA is issued in
+    // three parts, around the two halves of the table work, so that the
+    // instructions do not queue up all at once behind the memory pipeline;
+    // then wait for the DMA and the table writes, and barrier.
+    auto publish = [&](const Task& t, float4* xdst, const int b) __attribute__((always_inline)) {
+        if (t.row >= 0) {
+            const Dma d = plan(t.row, t.j0, t.j1, pass);
+            const int nmine = d.nins > s ? (d.nins - s + NS - 1) / NS : 0;  // this wave's instructions
+            const int k1 = nmine / 3, k2 = (2 * nmine) / 3;
+            for (int k = 0; k < k1; ++k) issue(d, xdst, s + k * NS);
+            tables(t, b, [&]() __attribute__((always_inline)) {
+                for (int k = k1; k < k2; ++k) issue(d, xdst, s + k * NS);
+            });
+            for (int k = k2; k < nmine; ++k) issue(d, xdst, s + k * NS);
+        }
+        if (s == 0 && lane == 0) s_rec[b] = make_int4(t.row, t.j0, t.j1, pass);
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA and table writes have landed
+        __syncthreads();
+    };
+    Task t = first_from_pass(0);
+    publish(t, s_xa, 0);
+    while (t.row >= 0) {
+        t = next_piece(t);
+        publish(t, s_xb, 1);
+        if (t.row < 0) break;
+        t = next_piece(t);
+        publish(t, s_xa, 0);
+    }
 }
 
 // Block order: consecutive workgroups (dispatched round-robin over the 8
@@ -609,18 +642,18 @@ __device__ __forceinline__ OpArgs item_args(const BatchItem& it)
     return g;
 }
 
-template <int P, int NW, int HL, int UMAX>
-__global__ __launch_bounds__(64 * NW) void forward_z0_kernel(OpArgs g)
+template <int P, int NW, int NS, int HL, int UMAX, int WPE>
+__global__ __launch_bounds__((FwdRoles<NW, NS>::THREADS), WPE) void forward_z0_kernel(OpArgs g)
 {
-    forward_block<P, NW, HL, UMAX>(g, xcd_logical(blockIdx.x, gridDim.x));
+    forward_block<P, NW, NS, HL, UMAX>(g, xcd_logical(blockIdx.x, gridDim.x));
 }
 
 // Persistent launch over a list of instances: block b handles work items
 // b, b + gridDim.x, ... of the concatenated (instance, block) list, in a
 // fixed order, so results do not depend on the grid or on the other
 // instances of the batch.
-template <int P, int NW, int HL, int UMAX>
-__global__ __launch_bounds__(64 * NW) void forward_batched_kernel(const BatchItem* __restrict__ items,
+template <int P, int NW, int NS, int HL, int UMAX, int WPE>
+__global__ __launch_bounds__((FwdRoles<NW, NS>::THREADS), WPE) void forward_batched_kernel(const BatchItem* __restrict__ items,
                                                                   int n, int64_t total)
 {
     for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
@@ -633,7 +666,7 @@ __global__ __launch_bounds__(64 * NW) void forward_batched_kernel(const BatchIte
             if (w < base + nbl) break;
             base += nbl;
         }
-        forward_block<P, NW, HL, UMAX>(item_args(items[i]), (int)(w - base));
+        forward_block<P, NW, NS, HL, UMAX>(item_args(items[i]), (int)(w - base));
         __syncthreads();
     }
 }
@@ -708,31 +741,30 @@ __global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict_
 
 template <int HL>
 struct FwdCfg;
-// 1 and 2 slices per lane: 12 angles per block (3 waves per SIMD) with
-// 28-pixel strips; against 8 angles with 32-pixel strips, 10-25 % faster on
-// most mid-size volumes (256^2 - 448^2, 64 - 128 slices), never more than
-// 2 % slower in the shapes measured.  Both variants share P, NW and UMAX, so
-// they add every output's terms in the same order (bitwise-equal results).
-template <> struct FwdCfg<1> { static constexpr int P = 28, NW = 12, UMAX = 128; };
-template <> struct FwdCfg<2> { static constexpr int P = 28, NW = 12, UMAX = 128; };
-// 4 slices per lane: 12 angles (waves) per block at 3 waves per SIMD (168
-// VGPRs; the few spills are outside the sweep) and 22-pixel strips.  Against
-// 8 waves with 28-pixel strips: 6 % faster per unit of work (1024^2 x 512 x
-// 1800), and at 512^3 x 192 the grid is exactly 3 x 256 workgroups instead of
-// 3.6 x 256 (6.2 vs 7.3 ms); 2-6 % slower at 256^3 and 384^3.  Pieces of up
-// to 76 voxels (152 KB of staging) split fewer lines than 72 (1.7 % faster).
-template <> struct FwdCfg<4> { static constexpr int P = 22, NW = 12, UMAX = 76; };
+// P pixels and NW angles per block, NS staging waves, pieces of up to UMAX
+// voxels; WPE is the occupancy (waves per SIMD) the register budget targets.
+// 1 and 2 slices per lane: 12 angles per block with 28-pixel strips.  Both
+// variants share P, NW, NS and UMAX, so they add every output's terms in the
+// same order (bitwise-equal results).
+template <> struct FwdCfg<1> { static constexpr int P = 28, NW = 12, NS = 4, UMAX = 128, WPE = 4; };
+template <> struct FwdCfg<2> { static constexpr int P = 28, NW = 12, NS = 4, UMAX = 128, WPE = 4; };
+// 4 slices per lane: 12 angles with 22-pixel strips (88 accumulators within
+// the 128 VGPRs of 16 waves per CU; the spills are fold addresses, outside
+// the sweep).  Pieces of up to 69 voxels: 140 KB of staging, 18 KB of slot
+// tables.
+template <> struct FwdCfg<4> { static constexpr int P = 22, NW = 12, NS = 4, UMAX = 69, WPE = 4; };
 
 // Narrower strips for 1 slice per lane, used by single launches of volumes
 // of at most UMAX = 128 voxels per side when the default grid would have fewer
 // workgroups than the device has CUs: twice the workgroups, each with half the
-// pixels (128^3 x 192: 0.46 -> 0.38 ms).  Within that size no line is split
+// pixels and 8 angles, two angles per staging wave, and a register budget for
+// six waves per SIMD (one block and part of the next per CU).  Within that size no line is split
 // into pieces and all lines fall in one fold group, so every output adds the
 // same terms in the same order as with the default strips: results are equal
 // bit for bit.  (For larger volumes the strip width changes where a line's staged
 // union is split and which pieces pair up around a fold, hence the last bits.)
 struct FwdSmall {
-    static constexpr int P = 16, NW = 8, UMAX = 128;
+    static constexpr int P = 16, NW = 8, NS = 4, UMAX = 128, WPE = 6;
 };
 
 template <int HL, typename C = FwdCfg<HL>>
@@ -742,8 +774,8 @@ hipError_t forward_single(const OpArgs& g, hipStream_t s)
     forward_grid<C::P, C::NW, HL>(g.A, g.H, g.PX, nbp, nba, nbh);
     const int64_t nb = (int64_t)nbp * nba * nbh;
     if (nb > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((forward_z0_kernel<C::P, C::NW, HL, C::UMAX>), dim3((unsigned)nb),
-                       dim3(64 * C::NW), 0, s, g);
+    hipLaunchKernelGGL((forward_z0_kernel<C::P, C::NW, C::NS, HL, C::UMAX, C::WPE>), dim3((unsigned)nb),
+                       dim3(FwdRoles<C::NW, C::NS>::THREADS), 0, s, g);
     return hipGetLastError();
 }
 
@@ -760,8 +792,8 @@ hipError_t forward_batched(const BatchItem* host, const BatchItem* dev, int n, i
     }
     if (total == 0) return hipSuccess;
     const int64_t gsz = grid > 0 ? std::min<int64_t>(grid, total) : total;
-    hipLaunchKernelGGL((forward_batched_kernel<C::P, C::NW, HL, C::UMAX>), dim3((unsigned)gsz),
-                       dim3(64 * C::NW), 0, s, dev, n, total);
+    hipLaunchKernelGGL((forward_batched_kernel<C::P, C::NW, C::NS, HL, C::UMAX, C::WPE>), dim3((unsigned)gsz),
+                       dim3(FwdRoles<C::NW, C::NS>::THREADS), 0, s, dev, n, total);
     return hipGetLastError();
 }
 
