@@ -382,6 +382,9 @@ __device__ __forceinline__ void backproject4_block(const OpArgs& g, const int lo
     static_assert(kTabAngles % APL == 0 && kTabAngles <= 64 * NW, "angle table range");
     __shared__ AngleK s_k[2][kTabAngles];
     __shared__ int s_rlo[2][kTabAngles];
+    // per wave, lane = voxel-angle of the current group: its two anchored
+    // weights, read back wave-uniformly by the tap loop (VGPR operands)
+    __shared__ float2 s_wt[NW][64];
 
     const float* __restrict__ y = g.in;
     float* __restrict__ x = g.out;
@@ -493,6 +496,9 @@ __device__ __forceinline__ void backproject4_block(const OpArgs& g, const int lo
         const float wr = (aval && t.l + 1 >= 0 && t.l + 1 < PX) ? t.wr : 0.0f;
         const float w3 = (aval && t.l + 2 >= 0 && t.l + 2 < PX) ? t.w3 : 0.0f;
         const uint64_t has3 = __ballot(w3 != 0.0f);
+        // (after the previous group's reads of s_wt: one wave's LDS accesses
+        // are performed in order)
+        s_wt[wave][lane] = make_float2(wl, wr);
         // Window row of pixel l.  For the tile's voxels the clamp never acts:
         // l - rlo <= floor(L) + 3 <= WIN - 2, and <= WIN - 3 when the third tap
         // is used (l = r - 1); it keeps the reads of voxels past the volume's
@@ -543,18 +549,24 @@ __device__ __forceinline__ void backproject4_block(const OpArgs& g, const int lo
             const int L0 = sub * NI;      // lanes L0 .. L0 + NI - 1 hold them
             const float* pn = tap_ptr(L0);
             T2 nxt = load2(pn);
+            // the two weights of item L, one item ahead: a wave-uniform LDS
+            // read gives them as VGPR operands (faster FMAs, no v_readlane)
+            const float2* wt = &s_wt[wave][0];
+            float2 cw_n = wt[L0];
 #pragma unroll
             for (int i = 0; i < NI; ++i) {
                 const int v = i % VPW;
                 const int L = L0 + i;
                 const float* pc = pn;
                 const T2 cur = nxt;
+                const float2 cw = cw_n;
                 if (i + 1 < NI) {
                     pn = tap_ptr(L + 1);
                     nxt = load2(pn);
+                    cw_n = wt[L + 1];
                 }
-                const float c0 = to_vgpr(rdlane(wl, L));
-                const float c1 = to_vgpr(rdlane(wr, L));
+                const float c0 = cw.x;
+                const float c1 = cw.y;
                 part[v][0] = fmaf(c0, cur.t[0].x, part[v][0]);
                 part[v][1] = fmaf(c0, cur.t[0].y, part[v][1]);
                 part[v][2] = fmaf(c0, cur.t[0].z, part[v][2]);
@@ -698,14 +710,16 @@ __device__ __forceinline__ int xcd_contiguous(int bid, int nb)
     return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
 }
 
+// Register budget for 4 waves per SIMD (128 VGPRs): two 8-wave workgroups
+// per CU, which hide each other's chunk barriers.
 template <int TX, int TY, int NW, int AC, int WIN, int TAB>
-__global__ __launch_bounds__(64 * NW) void backproject4_z0_kernel(OpArgs g)
+__global__ __launch_bounds__((64 * NW), 4) void backproject4_z0_kernel(OpArgs g)
 {
     backproject4_block<TX, TY, NW, AC, WIN, TAB>(g, xcd_contiguous(blockIdx.x, gridDim.x));
 }
 
 template <int TX, int TY, int NW, int AC, int WIN, int TAB>
-__global__ __launch_bounds__(64 * NW) void backproject4_batched_kernel(
+__global__ __launch_bounds__((64 * NW), 4) void backproject4_batched_kernel(
     const BatchItem* __restrict__ items, int n, int64_t total)
 {
     for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
