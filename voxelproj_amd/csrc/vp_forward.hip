@@ -10,9 +10,9 @@
 //           larger (|kappa| >= 1/sqrt 2), so consecutive voxels of a line
 //           are at least 1/sqrt 2 pixel apart on the detector and at most
 //           two of them share the first pixel l of their footprint;
-//   per line, lane m of a staging wave evaluates the footprint of the m-th
-//   voxel of the line segment that touches an angle's pixels (in
-//   increasing-u order) and files it in a slot table by its first pixel l;
+//   per line, lane q of a staging wave finds the (at most two) voxels whose
+//   footprint starts at pixel l = p0-2+q and files them in slot q of the
+//   angle's slot table (two angles per staging instruction stream);
 //   the angle's sweep wave then runs a statically unrolled sweep over
 //   l = p0-2 .. p0+P-1 that adds each voxel to accumulators l, l+1 and, for
 //   the one voxel-angle in eight whose footprint covers three pixels, l+2 --
@@ -36,29 +36,11 @@ constexpr int kSeg = 64;       // max voxels of one wave's line segment (one per
 constexpr int kRowsPerFold = 128;
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ float uni(float v)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
-}
 
 // One staged piece of work: voxels [j0, j1] of line `row` (-1 = none).
 struct Task {
     int row, j0, j1;
 };
-
-// Value of lane - 1 (lane 0: 0), by a DPP wavefront shift.
-__device__ __forceinline__ int lane_shift_up(int v)
-{
-    constexpr int kWaveShr1 = 0x138;
-    return __builtin_amdgcn_update_dpp(0, v, kWaveShr1, 0xf, 0xf, false);
-}
-
-// Value of lane + 1 (lane 63: 0), by a DPP wavefront shift.
-__device__ __forceinline__ int lane_shift_down(int v)
-{
-    constexpr int kWaveShl1 = 0x130;
-    return __builtin_amdgcn_update_dpp(0, v, kWaveShl1, 0xf, 0xf, false);
-}
 
 // Block = NW sweep waves (one angle each) x P detector pixels x 64*HL slices,
 // plus NS staging waves.  For every line of the volume along the angles'
@@ -126,7 +108,6 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     __shared__ int4 s_rec[2];
     __shared__ AngleK s_k[NW];
     __shared__ int s_mode[NW];
-    __shared__ float4 s_trash[64];  // discarded table writes, one slot per lane
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -393,19 +374,26 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     // that the sweep waves' FMAs would otherwise delay, holding up the barrier.
     __builtin_amdgcn_s_setprio(1);
     const int s = wave - NW;  // staging wave s builds the tables of angles s*AS .. s*AS+AS-1
-    // Lane w < NW describes angle w of the block.
+    // Lane w < NW describes angle w of the block (segment union of a line).
     const AngleK kl = s_k[lane < NW ? lane : 0];
     const int model = (lane < NW) ? s_mode[lane] : -1;
-    AngleK ka[AS];
-    int mode_a[AS];
+    // Slot-table lanes: lane = 32 * half + q builds slot q of angle 2 * pr + half
+    // of this wave's AS angles in table round pr, so one instruction stream
+    // serves two angles.
+    constexpr int ROUNDS = (AS + 1) / 2;
+    static_assert(SLOTS <= 32, "two angles' slots per wave");
+    const int half = lane >> 5;
+    const int q = lane & 31;
+    AngleK kv[ROUNDS];
+    int wv[ROUNDS], modev[ROUNDS];  // block angle of the lane per round (NW: none) and its pass
 #pragma unroll
-    for (int i = 0; i < AS; ++i) {
+    for (int pr = 0; pr < ROUNDS; ++pr) {
+        const int i = 2 * pr + half;
         const int w = s * AS + i;
-        const AngleK kk = s_k[w < NW ? w : 0];
-        ka[i].chi = uni(kk.chi); ka[i].clo = uni(kk.clo); ka[i].shi = uni(kk.shi); ka[i].slo = uni(kk.slo);
-        ka[i].a = uni(kk.a); ka[i].b = uni(kk.b); ka[i].h = uni(kk.h); ka[i].cd = uni(kk.cd);
-        ka[i].pch = uni(kk.pch); ka[i].plo = uni(kk.plo);
-        mode_a[i] = uni(w < NW ? s_mode[w] : -1);
+        const bool ok = i < AS && w < NW && q < SLOTS;
+        kv[pr] = s_k[ok ? w : 0];
+        wv[pr] = ok ? w : NW;
+        modev[pr] = ok ? s_mode[w] : -1;
     }
 
     // Pass state: lines run along x in pass 0 and along y in pass 1.
@@ -413,10 +401,9 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     bool px_ = true;
     int nrows = SY, nalong = SX;
     float ca = xc, cr = yc;
-    int64_t st_along = H, st_row = (int64_t)SX * H;
     float kp_l = 0.f, ik_l = 0.f, ot_l = 0.f, pc_l = 0.f;  // lane w < NW: angle w
-    float kap[AS], ikap[AS], oth[AS];
-    bool fwd[AS], mine[AS];
+    float kapv[ROUNDS], ikapv[ROUNDS], othv[ROUNDS], pcv[ROUNDS];  // the lane's angle per round
+    bool minev[ROUNDS];
     auto enter_pass = [&](int p) {
         pass = p;
         px_ = (p == 0);
@@ -424,20 +411,19 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
         nalong = px_ ? SX : SY;
         ca = px_ ? xc : yc;
         cr = px_ ? yc : xc;
-        st_along = px_ ? (int64_t)H : (int64_t)SX * H;
-        st_row = px_ ? (int64_t)SX * H : (int64_t)H;
         const float kcl = kl.chi + kl.clo, ksl = kl.shi + kl.slo;
         kp_l = px_ ? kcl : ksl;
         ot_l = px_ ? ksl : kcl;
         ik_l = 1.0f / kp_l;
         pc_l = kl.pch + kl.plo;  // pc - du of angle w (segment bounds only)
 #pragma unroll
-        for (int i = 0; i < AS; ++i) {
-            kap[i] = px_ ? (ka[i].chi + ka[i].clo) : (ka[i].shi + ka[i].slo);
-            oth[i] = px_ ? (ka[i].shi + ka[i].slo) : (ka[i].chi + ka[i].clo);
-            ikap[i] = 1.0f / kap[i];
-            fwd[i] = kap[i] >= 0.0f;  // the centre pixel grows along the line
-            mine[i] = (mode_a[i] == p);
+        for (int pr = 0; pr < ROUNDS; ++pr) {
+            const float kc = kv[pr].chi + kv[pr].clo, ks = kv[pr].shi + kv[pr].slo;
+            kapv[pr] = px_ ? kc : ks;
+            othv[pr] = px_ ? ks : kc;
+            ikapv[pr] = 1.0f / kapv[pr];
+            pcv[pr] = kv[pr].pch + kv[pr].plo;
+            minev[pr] = modev[pr] == p;
         }
     };
 
@@ -452,13 +438,10 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
         jhi = min(nalong - 1, (int)ceilf(fmaxf(ja, jb)) + 1);
     };
     // Line table for 64 consecutive lines c0 .. c0+63, lane = line: the union
-    // of the block's segments (what is staged) and the segments of this
-    // wave's angles; `nonempty` marks lines with anything to stage.
+    // of the block's segments (what is staged); `nonempty` marks lines with
+    // anything to stage.
     int c0 = -64;
     int u_lo = 0, u_hi = -1;
-    int s_lo[AS], s_hi[AS];
-#pragma unroll
-    for (int i = 0; i < AS; ++i) { s_lo[i] = 0; s_hi[i] = -1; }
     uint64_t nonempty = 0;
     auto load_lines = [&](int base) {
         c0 = base;
@@ -481,12 +464,6 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
         u_lo = lo;
         u_hi = valid ? hi : -1;
         nonempty = __ballot(valid && hi >= lo);
-#pragma unroll
-        for (int i = 0; i < AS; ++i) {
-            s_lo[i] = 0;
-            s_hi[i] = -1;
-            if (mine[i]) segment(R, kap[i], ikap[i], oth[i], ka[i].pch + ka[i].plo, s_lo[i], s_hi[i]);
-        }
     };
     // First piece at or after line `row` of the current pass (pieces hold at
     // most UMAX voxels).
@@ -527,69 +504,58 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
         return first_from_pass(pass + 1);
     };
 
-    // Slot tables of this wave's angles w = s*AS + i for piece t, in buffer
-    // b: footprints of each angle's voxels in the piece (lane = voxel), each
-    // written to slot (rank, q) for the rank-th voxel whose footprint starts
-    // at pixel p0 - 2 + q (at most two per pixel) as (wl, wr, w3, LDS offset;
-    // rank 1 without offset: it is rank 0's plus one voxel).  Empty slots hold
-    // zeros, so the sweep is straight-line code, and an angle with nothing
-    // to add from the piece gets an all-zero table.  Branch-free, so that the
-    // AS angles' dependency chains interleave: stores that are not wanted go
-    // to the lane's own word of s_trash.
+    // Slot tables of this wave's angles for piece t, in buffer b.  Slot q of
+    // an angle lists the voxels of the piece whose footprint starts at pixel
+    // l = p0 - 2 + q: (wl, wr, w3, LDS offset of its staged slices) for the
+    // first staged (rank 0), (wl, wr, w3, 0) for the one after it (rank 1:
+    // its slices follow rank 0's), zeros where there is none.  The first
+    // nonzero pixel of a footprint is floor(u - b + 1/2) (u its centre, b its
+    // half-width), so those voxels have u in [l + b - 1/2, l + b + 1/2):
+    // an interval of at most 1/|kappa| <= sqrt 2 voxels along the line, which
+    // holds at most two of them.  The lane solves for that interval (widened
+    // by 0.01 voxel against rounding), evaluates the footprints of its two
+    // candidate voxels and keeps those whose first pixel is l -- the same
+    // footprint function for every lane, so each voxel lands in exactly one
+    // slot.  Every slot of every angle of the block is written, so an angle
+    // with nothing to add from the piece gets an all-zero table.
     auto tables = [&](const Task& t, const int b, auto&& mid) __attribute__((always_inline)) {
         const float R = (float)t.row - cr;
-        int flag[AS];
-        Taps2 tp[AS];
-        int jlo[AS], jhi[AS], n[AS];
+        const int lq = pl - 1 + q;  // pixel of slot q
 #pragma unroll
-        for (int i = 0; i < AS; ++i) {
-            jlo[i] = uni(__builtin_amdgcn_readlane(s_lo[i], t.row - c0));
-            jhi[i] = uni(__builtin_amdgcn_readlane(s_hi[i], t.row - c0));
-            n[i] = (mine[i] && s * AS + i < NW) ? jhi[i] - jlo[i] + 1 : 0;
-            const int mcl = max(min(lane, n[i] - 1), 0);
-            const int jm = fwd[i] ? jlo[i] + mcl : jhi[i] - mcl;
-            // footprint anchored at its first nonzero pixel l; slot q = l - (pl - 1)
-            // covers l in [p0-2, p0+P-1], the voxels that touch the block's pixels
-            tp[i] = anchor_left(voxel_taps(px_ ? (float)jm - xc : R, px_ ? R : (float)jm - yc, ka[i]));
-        }
-        mid();
+        for (int pr = 0; pr < ROUNDS; ++pr) {
+            if (pr == 1) mid();
+            const AngleK& k = kv[pr];
+            const float ub = fmaf(-ca, kapv[pr], fmaf(R, othv[pr], pcv[pr]));  // u at j = 0
+            const float ulo = ((float)lq + k.b) - 0.5f;
+            const float ja = (ulo - ub) * ikapv[pr];
+            const float jb = ((ulo + 1.0f) - ub) * ikapv[pr];
+            // clamped before the conversion (lanes of angles outside this pass)
+            const float jf = fminf(fmaxf(ceilf(fminf(ja, jb) - 0.01f), -2.0f), (float)nalong + 2.0f);
+            const int j0 = (int)jf;
+            Taps2 c[2];
+            bool in[2];
 #pragma unroll
-        for (int i = 0; i < AS; ++i) {
-            const bool fw = fwd[i];
-            const int lprev = lane_shift_up(tp[i].l);    // lane m-1 holds the previous voxel
-            const int lnext = lane_shift_down(tp[i].l);  // lane m+1 the next
-            const bool valid = lane < n[i];
-            const int l = valid ? tp[i].l : 0x3fffffff;
-            const int mm0 = uni((int)__popcll(__ballot(valid && l < pl - 1)));
-            const int mend = uni((int)__popcll(__ballot(valid && l <= ph - 1)));
-            const int mA = max(fw ? t.j0 - jlo[i] : jhi[i] - t.j1, mm0);
-            const int mB = min(fw ? t.j1 - jlo[i] : jhi[i] - t.j0, mend - 1);
-            flag[i] = (n[i] > 0 && mA <= mB) ? 1 : 0;
-            const int jb0 = fw ? (jlo[i] - t.j0) : (jhi[i] - t.j0);  // staged slot of lane 0's voxel
-            const int dj = fw ? 1 : -1;
-            // Of two voxels with the same l (lanes m-1, m), rank 1 is the one
-            // staged later: lane m when j grows with the lane (fw), else lane m-1.
-            const bool in = flag[i] && lane >= mA && lane <= mB;
-            const bool second = fw ? (lane > mA && l == lprev) : (lane < mB && l == lnext);
-            const int q = l - (pl - 1);
-            const int off = (jb0 + dj * lane) * VOX;
-            const int w = min(s * AS + i, NW - 1);
-            // the whole table is zeroed also when the angle has nothing to add
-            const bool zf = s * AS + i < NW && lane < SLOTS;
-            float4* z0 = zf ? &s_tab[b][w][0][lane] : &s_trash[lane];
-            float4* z1 = zf ? &s_tab[b][w][1][lane] : &s_trash[lane];
-            float4* dst = in ? &s_tab[b][w][second ? 1 : 0][in ? q : 0] : &s_trash[lane];
-            const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-            *z0 = zero;
-            *z1 = zero;
-            // after the zero fill: one wave's LDS writes land in order
-            *dst = make_float4(tp[i].wl, tp[i].wr, tp[i].w3, second ? 0.f : __builtin_bit_cast(float, off));
+            for (int e = 0; e < 2; ++e) {
+                const int j = j0 + e;
+                const float jc = (float)j;
+                c[e] = anchor_left(voxel_taps(px_ ? jc - xc : R, px_ ? R : jc - yc, k));
+                in[e] = minev[pr] && j >= t.j0 && j <= t.j1 && c[e].l == lq;
+            }
+            if (wv[pr] < NW) {
+                const bool any = in[0] || in[1];
+                const Taps2 r0 = in[0] ? c[0] : c[1];
+                const int off = (j0 + (in[0] ? 0 : 1) - t.j0) * VOX;
+                const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+                s_tab[b][wv[pr]][0][q] = any ? make_float4(r0.wl, r0.wr, r0.w3, __builtin_bit_cast(float, off)) : zero;
+                s_tab[b][wv[pr]][1][q] = (in[0] && in[1]) ? make_float4(c[1].wl, c[1].wr, c[1].w3, 0.f) : zero;
+            }
         }
+        if (ROUNDS == 1) mid();
     };
     // Publish piece t (or the end marker) in buffer b: its DMA is issued in
-    // three parts, around the two halves of the table work, so that the
-    // instructions do not queue up all at once behind the memory pipeline;
-    // then wait for the DMA and the table writes, and barrier.
+    // three parts, around the table rounds, so that the instructions do not
+    // queue up all at once behind the memory pipeline; then wait for the DMA
+    // and the table writes, and barrier.
     auto publish = [&](const Task& t, float4* xdst, const int b) __attribute__((always_inline)) {
         if (t.row >= 0) {
             const Dma d = plan(t.row, t.j0, t.j1, pass);
