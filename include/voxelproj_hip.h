@@ -186,6 +186,30 @@ VP_API int vp_forward_batched(int variant, const vp_batch_item *items_host,
 VP_API int vp_backward_batched(int variant, const vp_batch_item *items_host,
                                const vp_batch_item *items_dev, int n, int grid, void *stream);
 
+/*
+ * Batched layout conversion, for batched launches of z_order 2 instances:
+ * out (+)= transpose(in) for n row-major matrices in one persistent launch
+ * (in: rows x cols; out: cols x rows; accumulate 1 adds to out, 0
+ * overwrites it).  A z_order 2 volume (H, SY, SX) is the H x SY*SX matrix
+ * whose transpose is the z_order 0 volume (SY, SX, H); a z_order 2 sinogram
+ * (H, A, PX) likewise transposes to (A, PX, H).  Descriptors are passed as
+ * for vp_forward_batched (host copy for validation, identical device copy
+ * read by the kernel); `grid` persistent workgroups, <= 0 for one per 64 x 64
+ * tile (at most 8192).  No counterpart in the reference, whose z_order 2
+ * kernels read their layout directly (voxelproj/csrc/include/sp_forward_z2.cuh,
+ * sp_backward_z2.cuh).
+ */
+typedef struct vp_transpose_item {
+    const float *in;
+    float *out;
+    int64_t rows, cols;
+    int32_t accumulate;
+    int32_t reserved;     /* 0 */
+} vp_transpose_item;
+
+VP_API int vp_transpose_batched(const vp_transpose_item *items_host,
+                                const vp_transpose_item *items_dev, int n, int grid, void *stream);
+
 /* Default kernel variant (slices per lane) of op 0 (A) / 1 (AT) for H slices,
  * the grouping key of batched launches.  vp_forward / vp_backward use it too,
  * except that 2 may drop to 1 for small and mid-size launches (fewer

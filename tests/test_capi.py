@@ -169,3 +169,25 @@ def test_noalloc_validation_before_device_work():
     assert b"scratch" in lib.vp_last_error()
     # empty problems are a no-op
     assert lib.vp_project_noalloc(0, 2, p, p, p, 0, 4, 4, 2, 5, None) == 0
+
+
+class _TransposeItem(ctypes.Structure):
+    _fields_ = [("inp", ctypes.c_void_p), ("out", ctypes.c_void_p), ("rows", ctypes.c_int64),
+                ("cols", ctypes.c_int64), ("accumulate", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+def test_transpose_batched_validates_before_device_work():
+    lib = voxelproj_amd.load()
+    p = ctypes.c_void_p(1)
+    ok = (_TransposeItem * 1)(_TransposeItem(1, 1, 3, 4, 0, 0))
+    assert lib.vp_transpose_batched(ok, ok, -1, 0, None) == -1       # negative count
+    assert lib.vp_transpose_batched(None, ok, 1, 0, None) == -1      # no host descriptors
+    assert lib.vp_transpose_batched(ok, None, 1, 0, None) == -1      # no device descriptors
+    assert lib.vp_transpose_batched(None, None, 0, 0, None) == 0     # nothing to do
+    for bad in (_TransposeItem(1, 1, -3, 4, 0, 0), _TransposeItem(None, 1, 3, 4, 0, 0),
+                _TransposeItem(1, 1, 3, 4, 2, 0), _TransposeItem(1, 1, 3, 4, 0, 7)):
+        arr = (_TransposeItem * 2)(_TransposeItem(1, 1, 2, 2, 1, 0), bad)
+        assert lib.vp_transpose_batched(arr, p, 2, 0, None) == -1
+        assert b"transpose item 1" in lib.vp_last_error()
+    huge = (_TransposeItem * 1)(_TransposeItem(1, 1, 1 << 30, 1 << 20, 0, 0))
+    assert lib.vp_transpose_batched(huge, p, 1, 0, None) == -2

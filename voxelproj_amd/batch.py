@@ -1,10 +1,13 @@
 """Batched projectors: many volumes of uneven size and angle count per launch.
 
-A ``BatchPlan`` holds the descriptors of a list of z_order-0 instances (host
-copy for validation, device copy for the kernel) and launches them as one
-persistent grouped kernel through ``vp_forward_batched`` /
-``vp_backward_batched``.  Plans only read device memory when launched, so an
-A plan followed by an AT plan can be captured into a HIP graph
+A ``BatchPlan`` holds the descriptors of a list of instances (host copy for
+validation, device copy for the kernel) and launches them as one persistent
+grouped kernel through ``vp_forward_batched`` / ``vp_backward_batched``.
+The batched kernels read the z_order 0 layout (H fastest); a plan of
+z_order 2 instances keeps slice-fastest copies of its inputs and outputs and
+wraps the kernel in two batched layout transposes (``vp_transpose_batched``),
+as the single-instance entry points do.  Plans only read device memory when
+launched, so an A plan followed by an AT plan can be captured into a HIP graph
 (``capture_pair``) and replayed.
 
 The kernel variant (H slices per lane, which fixes the tile shapes) is
@@ -34,6 +37,29 @@ FORWARD_VARIANTS = (1, 2, 4)
 BACKWARD_VARIANTS = (1, 2, 4)
 
 
+class _TItem(ctypes.Structure):
+    _fields_ = [
+        ("in_", ctypes.c_void_p),
+        ("out", ctypes.c_void_p),
+        ("rows", ctypes.c_int64),
+        ("cols", ctypes.c_int64),
+        ("accumulate", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+def _to_device(arr, device):
+    """A ctypes descriptor array as (pinned host copy, device copy): the
+    upload is stream-ordered from pinned memory, so building a plan while
+    earlier work is in flight does not wait for the device."""
+    raw = np.frombuffer(bytes(arr), dtype=np.uint8)
+    pinned = torch.empty(raw.size, dtype=torch.uint8, pin_memory=True)
+    pinned.numpy()[:] = raw
+    with torch.cuda.device(device):
+        dev = pinned.to(device, non_blocking=True)
+    return pinned, dev
+
+
 class _Item(ctypes.Structure):
     _fields_ = [
         ("in_", ctypes.c_void_p),
@@ -55,17 +81,24 @@ def default_grid(device=None):
 
 
 class BatchPlan:
-    """One batched launch of A (op='A') or AT (op='AT') over z_order-0 instances.
+    """One batched launch of A (op='A') or AT (op='AT') over instances of one
+    layout.
 
     ins / outs: lists of contiguous float32 device tensors; angles: list of
-    float32 device tensors.  For A, ins are volumes (SY, SX, H) and outs
-    sinograms (A, PX, H); for AT the other way round.  accumulate=False
-    overwrites the outputs.
+    float32 device tensors.  For A, ins are volumes and outs sinograms; for
+    AT the other way round; z_order 0: volumes (SY, SX, H), sinograms
+    (A, PX, H); z_order 2: (H, SY, SX) and (H, A, PX).  accumulate=False
+    overwrites the outputs.  A z_order 2 plan holds slice-fastest workspaces
+    the size of its inputs and outputs (from torch's allocator, kept for the
+    plan's lifetime).
     """
 
-    def __init__(self, op, ins, outs, angles, accumulate=False, variant=None, grid=None):
+    def __init__(self, op, ins, outs, angles, accumulate=False, variant=None, grid=None, z_order=0):
         if op not in ("A", "AT"):
             raise ValueError("op must be 'A' or 'AT'")
+        if z_order not in (0, 2):
+            raise ValueError(f"z_order must be 0 or 2, got {z_order}")
+        self.z_order = z_order
         if not (len(ins) == len(outs) == len(angles)) or len(ins) == 0:
             raise ValueError("ins, outs and angles must be non-empty lists of equal length")
         self.op = op
@@ -74,7 +107,10 @@ class BatchPlan:
         self.angles = [ops.device_angles(a, self.device) for a in angles]
         n = len(ins)
         arr = (_Item * n)()
+        t_in, t_out = (_TItem * n)(), (_TItem * n)()
         self.shapes = []
+        self._ws = []
+        acc = 1 if accumulate else 0
         for i, (a, b, ang) in enumerate(zip(self.ins, self.outs, self.angles)):
             for t, nm in ((a, "input"), (b, "output")):
                 if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and t.dim() == 3):
@@ -82,25 +118,31 @@ class BatchPlan:
                 if t.device != self.device:
                     raise ValueError(f"instance {i}: all tensors must be on {self.device}")
             if op == "A":
-                SY, SX, H = a.shape
-                A_, PX, H2 = b.shape
+                SY, SX, H = ops.vol_dims(a.shape, z_order)
+                A_, PX, H2 = ops.sino_dims(b.shape, z_order)
             else:
-                A_, PX, H = a.shape
-                SY, SX, H2 = b.shape
+                A_, PX, H = ops.sino_dims(a.shape, z_order)
+                SY, SX, H2 = ops.vol_dims(b.shape, z_order)
             if H2 != H or A_ != ang.numel():
                 raise ValueError(f"instance {i}: inconsistent shapes {tuple(a.shape)} / {tuple(b.shape)} "
                                  f"with {ang.numel()} angles")
-            arr[i] = _Item(a.data_ptr(), b.data_ptr(), ang.data_ptr(), A_, SY, SX, H, PX,
-                           1 if accumulate else 0)
+            if z_order == 0:
+                arr[i] = _Item(a.data_ptr(), b.data_ptr(), ang.data_ptr(), A_, SY, SX, H, PX, acc)
+            else:
+                # (H, rest) -> (rest, H) into the workspace, the kernel on the
+                # workspaces, then (rest, H) -> (H, rest) added into the output
+                wi = torch.empty((a.numel() // H, H), device=self.device)
+                wo = torch.empty((b.numel() // H, H), device=self.device)
+                self._ws += [wi, wo]
+                arr[i] = _Item(wi.data_ptr(), wo.data_ptr(), ang.data_ptr(), A_, SY, SX, H, PX, 0)
+                t_in[i] = _TItem(a.data_ptr(), wi.data_ptr(), H, a.numel() // H, 0, 0)
+                t_out[i] = _TItem(wo.data_ptr(), b.data_ptr(), b.numel() // H, H, acc, 0)
             self.shapes.append((int(SY), int(SX), int(H), int(A_), int(PX)))
         self._host = arr
-        # descriptor upload: from pinned memory, stream-ordered, so building a
-        # plan while earlier work is in flight does not wait for the device
-        raw = np.frombuffer(bytes(arr), dtype=np.uint8)
-        self._pinned = torch.empty(raw.size, dtype=torch.uint8, pin_memory=True)
-        self._pinned.numpy()[:] = raw
-        with torch.cuda.device(self.device):
-            self._dev = self._pinned.to(self.device, non_blocking=True)
+        self._pinned, self._dev = _to_device(arr, self.device)
+        if z_order == 2:
+            self._t_host = (t_in, t_out)
+            self._t_dev = (_to_device(t_in, self.device), _to_device(t_out, self.device))
         self.n = n
         self.grid = default_grid(self.device) if grid is None else int(grid)
         self.variant = variant if variant is not None else _default_variant(op, self.shapes)
@@ -111,7 +153,13 @@ class BatchPlan:
         return sum(SY * SX * H * A for SY, SX, H, A, _ in self.shapes)
 
     def signature(self):
-        return f"{self.op}:" + ";".join(",".join(map(str, s)) for s in sorted(self.shapes))
+        return f"{self.op}:z{self.z_order}:" + ";".join(",".join(map(str, s)) for s in sorted(self.shapes))
+
+    def _transpose(self, k, stream):
+        lib = _lib.load()
+        rc = lib.vp_transpose_batched(ctypes.addressof(self._t_host[k]), self._t_dev[k][1].data_ptr(),
+                                      self.n, 0, stream)
+        _lib.check(rc, "batched layout transpose")
 
     def launch(self, variant=None):
         lib = _lib.load()
@@ -119,9 +167,13 @@ class BatchPlan:
         fn = lib.vp_forward_batched if self.op == "A" else lib.vp_backward_batched
         with torch.cuda.device(self.device):
             stream = torch.cuda.current_stream(self.device).cuda_stream
+            if self.z_order == 2:
+                self._transpose(0, stream)
             rc = fn(int(v), ctypes.addressof(self._host), self._dev.data_ptr(), self.n,
                     self.grid, stream)
-        _lib.check(rc, "batched " + self.op)
+            _lib.check(rc, "batched " + self.op)
+            if self.z_order == 2:
+                self._transpose(1, stream)
         return self.outs
 
 
@@ -206,20 +258,24 @@ class GroupedPlan:
     persistent launch.  ``variant_of(H)`` overrides the grouping rule.
     """
 
-    def __init__(self, op, ins, outs, angles, accumulate=False, grid=None, variant_of=None):
+    def __init__(self, op, ins, outs, angles, accumulate=False, grid=None, variant_of=None,
+                 z_order=0):
         if not (len(ins) == len(outs) == len(angles)) or len(ins) == 0:
             raise ValueError("ins, outs and angles must be non-empty lists of equal length")
+        if z_order not in (0, 2):
+            raise ValueError(f"z_order must be 0 or 2, got {z_order}")
         lib = _lib.load()
         code = 0 if op == "A" else 1
         groups = {}
         for i, t in enumerate(ins):
-            H = int(t.shape[2])  # z_order 0: H is the last extent of volumes and sinograms
+            # H: the last extent of volumes and sinograms in z_order 0, the first in 2
+            H = int(t.shape[2] if z_order == 0 else t.shape[0])
             v = int(variant_of(H)) if variant_of is not None else int(lib.vp_default_variant(code, H))
             groups.setdefault(v, []).append(i)
         self.op = op
         self.device = ins[0].device
         self.plans = [BatchPlan(op, [ins[i] for i in idx], [outs[i] for i in idx],
-                                [angles[i] for i in idx], accumulate, v, grid)
+                                [angles[i] for i in idx], accumulate, v, grid, z_order)
                       for v, idx in sorted(groups.items())]
         self.outs = list(outs)
         self._streams = None
@@ -314,27 +370,28 @@ def uneven_workload(n=48, seed=0, device="cuda"):
     return out
 
 
-def forward_batch(xs, angles, ys=None, PX=None, accumulate=False, variant=None, grid=None):
+def forward_batch(xs, angles, ys=None, PX=None, accumulate=False, variant=None, grid=None,
+                  z_order=0):
     """A for a list of volumes in one launch; returns the list of sinograms."""
     if ys is None:
         ys = []
         for x, a in zip(xs, angles):
-            SY, SX, H = x.shape
+            SY, SX, H = ops.vol_dims(x.shape, z_order)
             px = max(SY, SX) if PX is None else PX
-            ys.append(torch.empty((ops.device_angles(a, x.device).numel(), px, H),
+            ys.append(torch.empty(ops.sino_shape(ops.device_angles(a, x.device).numel(), px, H, z_order),
                                   device=x.device, dtype=torch.float32))
         accumulate = False
-    BatchPlan("A", xs, ys, angles, accumulate, variant, grid).launch()
+    BatchPlan("A", xs, ys, angles, accumulate, variant, grid, z_order).launch()
     return ys
 
 
-def backward_batch(ys, angles, xs=None, accumulate=False, variant=None, grid=None):
+def backward_batch(ys, angles, xs=None, accumulate=False, variant=None, grid=None, z_order=0):
     """AT for a list of sinograms in one launch; returns the list of volumes."""
     if xs is None:
         xs = []
         for y, a in zip(ys, angles):
-            A_, PX, H = y.shape
-            xs.append(torch.empty((PX, PX, H), device=y.device, dtype=torch.float32))
+            A_, PX, H = ops.sino_dims(y.shape, z_order)
+            xs.append(torch.empty(ops.vol_shape(PX, PX, H, z_order), device=y.device, dtype=torch.float32))
         accumulate = False
-    BatchPlan("AT", ys, xs, angles, accumulate, variant, grid).launch()
+    BatchPlan("AT", ys, xs, angles, accumulate, variant, grid, z_order).launch()
     return xs

@@ -61,6 +61,11 @@ static_assert(offsetof(vp_batch_item, n_angles) == offsetof(vp::BatchItem, A), "
 static_assert(offsetof(vp_batch_item, accumulate) == offsetof(vp::BatchItem, accumulate),
               "batch item layout");
 
+static_assert(sizeof(vp_transpose_item) == sizeof(vp::TransposeItem), "transpose item layout");
+static_assert(offsetof(vp_transpose_item, rows) == offsetof(vp::TransposeItem, R), "transpose item layout");
+static_assert(offsetof(vp_transpose_item, accumulate) == offsetof(vp::TransposeItem, accumulate),
+              "transpose item layout");
+
 int validate_batch(int op, int variant, const vp_batch_item* items, int n)
 {
     if (n < 0) return fail(VP_ERR_ARG, "negative batch size");
@@ -557,6 +562,31 @@ int vp_forward_batched(int variant, const vp_batch_item* items_host, const vp_ba
         variant, reinterpret_cast<const vp::BatchItem*>(items_host),
         reinterpret_cast<const vp::BatchItem*>(items_dev), n, grid, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "batched forward launch");
+    return VP_OK;
+}
+
+int vp_transpose_batched(const vp_transpose_item* items_host, const vp_transpose_item* items_dev,
+                         int n, int grid, void* stream)
+{
+    g_last_error.clear();
+    if (n < 0) return fail(VP_ERR_ARG, "negative batch size");
+    if (n == 0) return VP_OK;
+    if (items_host == nullptr || items_dev == nullptr) return fail(VP_ERR_ARG, "null descriptor array");
+    for (int i = 0; i < n; ++i) {
+        const vp_transpose_item& it = items_host[i];
+        const std::string who = "transpose item " + std::to_string(i);
+        if (it.rows < 0 || it.cols < 0) return fail(VP_ERR_ARG, who + ": negative extent");
+        if (it.rows > ((int64_t)1 << 40) / std::max<int64_t>(it.cols, 1))
+            return fail(VP_ERR_LIMIT, who + ": matrix too large");
+        if (it.rows * it.cols > 0 && (it.in == nullptr || it.out == nullptr))
+            return fail(VP_ERR_ARG, who + ": null device pointer");
+        if (it.accumulate != 0 && it.accumulate != 1) return fail(VP_ERR_ARG, who + ": accumulate must be 0 or 1");
+        if (it.reserved != 0) return fail(VP_ERR_ARG, who + ": reserved field must be 0");
+    }
+    hipError_t e = vp::launch_transpose_batched(reinterpret_cast<const vp::TransposeItem*>(items_host),
+                                                reinterpret_cast<const vp::TransposeItem*>(items_dev), n, grid,
+                                                (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "batched transpose launch");
     return VP_OK;
 }
 

@@ -639,69 +639,100 @@ __global__ __launch_bounds__((FwdRoles<NW, NS>::THREADS), WPE) void forward_batc
 
 constexpr int kTile = 64;
 
+// One 64 x 64 tile of out[c][r] (+)= in[r][c] for an R x C row-major
+// matrix, by a 256-thread workgroup through an LDS tile: 16-byte loads and
+// stores on full tiles when both extents are multiples of 4 and both
+// pointers 16-byte aligned (an accumulating store loads all its old values
+// first), element-wise on ragged edges.  Ends with a barrier, so the tile
+// buffer can be reused at once.
+__device__ __forceinline__ void transpose_tile(const float* __restrict__ in, float* __restrict__ out,
+                                               int64_t R, int64_t C, int accumulate, int64_t tile,
+                                               float (&t)[kTile][kTile + 1])
+{
+    const int64_t tiles_c = (C + kTile - 1) / kTile;
+    const bool vec = ((R | C) & 3) == 0 && ((reinterpret_cast<uintptr_t>(in) |
+                                             reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // scalar path
+    const int lr = threadIdx.x >> 4, lc = (threadIdx.x & 15) * 4;  // vector path
+    const int64_t r0 = (tile / tiles_c) * kTile, c0 = (tile % tiles_c) * kTile;
+    if (vec && r0 + kTile <= R && c0 + kTile <= C) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int i = lr + 16 * k;
+            const float4 v = *reinterpret_cast<const float4*>(in + (r0 + i) * C + c0 + lc);
+            t[i][lc] = v.x;
+            t[i][lc + 1] = v.y;
+            t[i][lc + 2] = v.z;
+            t[i][lc + 3] = v.w;
+        }
+        __syncthreads();
+        float4 o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int j = lr + 16 * k;
+            o[k] = make_float4(t[lc][j], t[lc + 1][j], t[lc + 2][j], t[lc + 3][j]);
+        }
+        if (accumulate) {
+            float4 old[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                old[k] = *reinterpret_cast<const float4*>(out + (c0 + lr + 16 * k) * R + r0 + lc);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                o[k].x += old[k].x;
+                o[k].y += old[k].y;
+                o[k].z += old[k].z;
+                o[k].w += old[k].w;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            *reinterpret_cast<float4*>(out + (c0 + lr + 16 * k) * R + r0 + lc) = o[k];
+    } else {
+        for (int i = ty; i < kTile; i += 4) {
+            const int64_t r = r0 + i, c = c0 + tx;
+            t[i][tx] = (r < R && c < C) ? in[r * C + c] : 0.0f;
+        }
+        __syncthreads();
+        for (int i = ty; i < kTile; i += 4) {
+            const int64_t c = c0 + i, r = r0 + tx;
+            if (r < R && c < C) {
+                float* po = out + c * R + r;
+                *po = accumulate ? (*po + t[tx][i]) : t[tx][i];
+            }
+        }
+    }
+    __syncthreads();  // the tile buffer is reused by the next tile
+}
+
+__host__ __device__ inline int64_t transpose_tiles(int64_t R, int64_t C)
+{
+    return ((R + kTile - 1) / kTile) * ((C + kTile - 1) / kTile);
+}
+
 __global__ __launch_bounds__(256) void transpose_kernel(const float* __restrict__ in,
                                                         float* __restrict__ out, int64_t R,
                                                         int64_t C, int accumulate)
 {
     __shared__ float t[kTile][kTile + 1];
-    const int64_t tiles_r = (R + kTile - 1) / kTile, tiles_c = (C + kTile - 1) / kTile;
-    // 16-byte rows: both extents multiples of 4 and both pointers aligned
-    const bool vec = ((R | C) & 3) == 0 && ((reinterpret_cast<uintptr_t>(in) |
-                                             reinterpret_cast<uintptr_t>(out)) & 15) == 0;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;   // scalar path
-    const int lr = threadIdx.x >> 4, lc = (threadIdx.x & 15) * 4;  // vector path
-    for (int64_t tile = blockIdx.x; tile < tiles_r * tiles_c; tile += gridDim.x) {
-        const int64_t r0 = (tile / tiles_c) * kTile, c0 = (tile % tiles_c) * kTile;
-        if (vec && r0 + kTile <= R && c0 + kTile <= C) {
-            // full tile: 16-byte loads and stores, all old values of an
-            // accumulating store loaded before any is written
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int i = lr + 16 * k;
-                const float4 v = *reinterpret_cast<const float4*>(in + (r0 + i) * C + c0 + lc);
-                t[i][lc] = v.x;
-                t[i][lc + 1] = v.y;
-                t[i][lc + 2] = v.z;
-                t[i][lc + 3] = v.w;
-            }
-            __syncthreads();
-            float4 o[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int j = lr + 16 * k;
-                o[k] = make_float4(t[lc][j], t[lc + 1][j], t[lc + 2][j], t[lc + 3][j]);
-            }
-            if (accumulate) {
-                float4 old[4];
-#pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    old[k] = *reinterpret_cast<const float4*>(out + (c0 + lr + 16 * k) * R + r0 + lc);
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    o[k].x += old[k].x;
-                    o[k].y += old[k].y;
-                    o[k].z += old[k].z;
-                    o[k].w += old[k].w;
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                *reinterpret_cast<float4*>(out + (c0 + lr + 16 * k) * R + r0 + lc) = o[k];
-        } else {
-            for (int i = ty; i < kTile; i += 4) {
-                const int64_t r = r0 + i, c = c0 + tx;
-                t[i][tx] = (r < R && c < C) ? in[r * C + c] : 0.0f;
-            }
-            __syncthreads();
-            for (int i = ty; i < kTile; i += 4) {
-                const int64_t c = c0 + i, r = r0 + tx;
-                if (r < R && c < C) {
-                    float* po = out + c * R + r;
-                    *po = accumulate ? (*po + t[tx][i]) : t[tx][i];
-                }
-            }
+    const int64_t n = transpose_tiles(R, C);
+    for (int64_t tile = blockIdx.x; tile < n; tile += gridDim.x) transpose_tile(in, out, R, C, accumulate, tile, t);
+}
+
+// Persistent launch over the concatenated tile lists of n matrices.
+__global__ __launch_bounds__(256) void transpose_batched_kernel(const TransposeItem* __restrict__ items,
+                                                                int n, int64_t total)
+{
+    __shared__ float t[kTile][kTile + 1];
+    for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
+        int i = 0;
+        int64_t base = 0;
+        for (; i < n - 1; ++i) {
+            const int64_t nt = transpose_tiles(items[i].R, items[i].C);
+            if (w < base + nt) break;
+            base += nt;
         }
-        __syncthreads();  // the tile buffer is reused by the next tile
+        transpose_tile(items[i].in, items[i].out, items[i].R, items[i].C, items[i].accumulate, w - base, t);
     }
 }
 
@@ -819,10 +850,21 @@ hipError_t launch_transpose(const float* in, float* out, int64_t R, int64_t C, i
 {
     // workgroups stride over the tiles (one-dimensional grid: no limit on
     // either extent's tile count)
-    const int64_t tiles = ((R + kTile - 1) / kTile) * ((C + kTile - 1) / kTile);
+    const int64_t tiles = transpose_tiles(R, C);
     if (tiles == 0) return hipSuccess;
     const unsigned grid = (unsigned)std::min<int64_t>(tiles, 8192);
     hipLaunchKernelGGL(transpose_kernel, dim3(grid), dim3(256), 0, s, in, out, R, C, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose_batched(const TransposeItem* host, const TransposeItem* dev, int n,
+                                    int grid, hipStream_t s)
+{
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) total += transpose_tiles(host[i].R, host[i].C);
+    if (total == 0) return hipSuccess;
+    const unsigned g = (unsigned)std::min<int64_t>(grid > 0 ? grid : total, std::min<int64_t>(total, 8192));
+    hipLaunchKernelGGL(transpose_batched_kernel, dim3(g), dim3(256), 0, s, dev, n, total);
     return hipGetLastError();
 }
 

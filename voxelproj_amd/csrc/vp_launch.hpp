@@ -87,4 +87,18 @@ hipError_t launch_fill(T* p, int64_t n, T v, hipStream_t s);
 hipError_t launch_transpose(const float* in, float* out, int64_t R, int64_t C, int accumulate,
                             hipStream_t s);
 
+// One matrix of a batched transpose (layout shared with the C ABI's
+// vp_transpose_item).
+struct TransposeItem {
+    const float* in;
+    float* out;
+    int64_t R, C;
+    int32_t accumulate, reserved;
+};
+
+// out[c][r] (+)= in[r][c] for each of n matrices in one persistent launch;
+// items_dev holds the same descriptors as items_host.
+hipError_t launch_transpose_batched(const TransposeItem* items_host, const TransposeItem* items_dev,
+                                    int n, int grid, hipStream_t s);
+
 }  // namespace vp

@@ -27,6 +27,10 @@ A request's completion time is read off the GPU
 clock: the elapsed time from an event recorded at the start of the run to
 the event after its last step.
 
+Requests come in either layout (z_order 0: volumes (SY, SX, H), sinograms
+(A, PX, H); z_order 2: (H, SY, SX) and (H, A, PX)); only requests of the
+same layout are batched together.
+
 Which path a request takes depends only on its own shape, and the batched
 kernels add every output's terms in the same order as the single launches,
 so a request's result does not depend on what else is in flight: alone and
@@ -56,11 +60,12 @@ class Request:
     rid: int
     kind: str                  # "A", "AT" or "SIRT"
     arrival: float             # seconds from the start of the run
-    data: torch.Tensor         # A: volume (SY,SX,H); AT: sinogram (A,PX,H); SIRT: sinogram b
+    data: torch.Tensor         # A: volume; AT: sinogram; SIRT: sinogram b (in layout z_order)
     angles: torch.Tensor
     SY: int
     SX: int
     n_iter: int = 10
+    z_order: int = 0
     # filled in by the server
     result: torch.Tensor = None
     t_admit: float = None
@@ -72,14 +77,28 @@ class Request:
             raise ValueError(f"request kind must be one of {KINDS}, got {self.kind!r}")
         if self.kind == "SIRT" and int(self.n_iter) < 0:
             raise ValueError("a SIRT request needs n_iter >= 0")
+        if self.z_order not in (0, 2):
+            raise ValueError(f"z_order must be 0 or 2, got {self.z_order}")
 
     @property
     def H(self):
-        return int(self.data.shape[2])
+        if self.kind == "A":
+            return ops.vol_dims(self.data.shape, self.z_order)[2]
+        return ops.sino_dims(self.data.shape, self.z_order)[2]
 
     @property
     def PX(self):
-        return int(self.data.shape[1]) if self.kind != "A" else max(self.SY, self.SX)
+        if self.kind == "A":
+            return max(self.SY, self.SX)
+        return ops.sino_dims(self.data.shape, self.z_order)[1]
+
+    @property
+    def vshape(self):
+        return ops.vol_shape(self.SY, self.SX, self.H, self.z_order)
+
+    @property
+    def sshape(self):
+        return ops.sino_shape(self.n_angles, self.PX, self.H, self.z_order)
 
     @property
     def n_angles(self):
@@ -117,11 +136,11 @@ def _run_chain(r):
     """The plain single-instance operators for one request (on the current
     stream); returns its result."""
     if r.kind == "A":
-        return ops.project(r.data, r.angles, r.PX)
+        return ops.project(r.data, r.angles, r.PX, r.z_order)
     if r.kind == "AT":
-        return ops.backproject(r.data, r.angles, r.SY, r.SX)
-    solver = recon.SIRT(r.angles, r.SY, r.SX, r.H, r.PX, device=r.data.device)
-    x = torch.zeros((r.SY, r.SX, r.H), device=r.data.device)
+        return ops.backproject(r.data, r.angles, r.SY, r.SX, r.z_order)
+    solver = recon.SIRT(r.angles, r.SY, r.SX, r.H, r.PX, z_order=r.z_order, device=r.data.device)
+    x = torch.zeros(r.vshape, device=r.data.device)
     solver.iterate(x, r.data, r.n_iter)
     return x
 
@@ -197,13 +216,13 @@ class Server:
         steps = deque()
         with torch.cuda.stream(stream):
             if req.kind == "A":
-                req.result = torch.empty((req.n_angles, req.PX, req.H), device=dev)
+                req.result = torch.empty(req.sshape, device=dev)
                 steps.append(("A", req.data, req.result, None))
             elif req.kind == "AT":
-                req.result = torch.empty((req.SY, req.SX, req.H), device=dev)
+                req.result = torch.empty(req.vshape, device=dev)
                 steps.append(("AT", req.data, req.result, None))
             else:
-                vshape = (req.SY, req.SX, req.H)
+                vshape = req.vshape
                 sshape = tuple(req.data.shape)
                 x = torch.zeros(vshape, device=dev)
                 R = torch.empty(sshape, device=dev)
@@ -226,14 +245,15 @@ class Server:
         st["remaining"] = req.unit_work() * len(steps)
 
     # ------------------------------------------------------------ launches
-    def _plan(self, op, v, jobs):
-        """The batched launch of `jobs` [(in, out, angles)] under variant v,
-        built once per set of buffers and reused while that set recurs."""
-        key = (op, v) + tuple((j[0].data_ptr(), j[1].data_ptr(), j[2].data_ptr()) for j in jobs)
+    def _plan(self, op, v, z, jobs):
+        """The batched launch of `jobs` [(in, out, angles)] of layout z under
+        variant v, built once per set of buffers and reused while that set
+        recurs."""
+        key = (op, v, z) + tuple((j[0].data_ptr(), j[1].data_ptr(), j[2].data_ptr()) for j in jobs)
         plan = self._plans.get(key)
         if plan is None:
             plan = BatchPlan(op, [j[0] for j in jobs], [j[1] for j in jobs], [j[2] for j in jobs],
-                             accumulate=False, variant=v, grid=self.grid)
+                             accumulate=False, variant=v, grid=self.grid, z_order=z)
             self._plans[key] = plan
             while len(self._plans) > 64:
                 self._plans.popitem(last=False)
@@ -244,9 +264,9 @@ class Server:
     def _launch_single(self, r, op, inp, out):
         out.zero_()
         if op == "A":
-            ops.project(inp, r.angles, out.shape[1], out=out)
+            ops.project(inp, r.angles, r.PX, r.z_order, out=out)
         else:
-            ops.backproject(inp, r.angles, out.shape[0], out.shape[1], out=out)
+            ops.backproject(inp, r.angles, r.SY, r.SX, r.z_order, out=out)
 
     def tick(self, lane=None):
         """Submit one step on one stream (by default the first stream whose
@@ -275,9 +295,9 @@ class Server:
             if len(group) > 1:
                 by_variant = {}
                 for r, (_, inp, out, _post) in zip(group, steps):
-                    by_variant.setdefault(_variant(op, r.H), []).append((inp, out, r.angles))
-                for v, jobs in sorted(by_variant.items()):
-                    self._plan(op, v, jobs).launch()
+                    by_variant.setdefault((_variant(op, r.H), r.z_order), []).append((inp, out, r.angles))
+                for (v, z), jobs in sorted(by_variant.items()):
+                    self._plan(op, v, z, jobs).launch()
             else:
                 _, inp, out, _post = steps[0]
                 self._launch_single(lead, op, inp, out)
@@ -367,7 +387,8 @@ class Server:
 def run_alone(req, device="cuda"):
     """The same request through the same kernels with nothing else in flight."""
     s = Server(device=device)
-    clone = Request(req.rid, req.kind, 0.0, req.data, req.angles, req.SY, req.SX, req.n_iter)
+    clone = Request(req.rid, req.kind, 0.0, req.data, req.angles, req.SY, req.SX, req.n_iter,
+                    req.z_order)
     s.run([clone])
     return clone.result
 
@@ -410,9 +431,11 @@ def run_plain_fifo(requests):
 
 
 def mixed_workload(n, rate=200.0, seed=0, sizes=(128, 256, 512), heights=(64, 128, 256, 512),
-                   n_angles=(90, 192, 360, 720, 1440), n_iter=10, device="cuda"):
+                   n_angles=(90, 192, 360, 720, 1440), n_iter=10, device="cuda", layouts=(0,)):
     """Poisson request stream: 40% A, 40% AT, 20% SIRT; ellipsoid-phantom or
-    N(0,1) inputs generated on the device (seeded)."""
+    N(0,1) inputs generated on the device (seeded); each request's layout
+    drawn from `layouts` (the default draws nothing more from the generator,
+    so the z_order 0 stream is the same as without layouts)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     reqs = []
@@ -433,7 +456,10 @@ def mixed_workload(n, rate=200.0, seed=0, sizes=(128, 256, 512), heights=(64, 12
         else:
             x_true = recon.ellipsoid_phantom(S, S, H, seed=seed * 100003 + i, device=device)
             data = ops.project(x_true, ang, S)
-        reqs.append(Request(i, str(kind), t, data, ang, S, S, n_iter))
+        z = int(rng.choice(layouts)) if len(layouts) > 1 else int(layouts[0])
+        if z == 2:
+            data = data.permute(2, 0, 1).contiguous()
+        reqs.append(Request(i, str(kind), t, data, ang, S, S, n_iter, z))
     return reqs
 
 
