@@ -166,9 +166,9 @@ VP_API int vp_vshift(int z_order, int transpose, const float *in, float *out, in
  * Each instance is described by a vp_batch_item.  The caller passes the
  * descriptor array twice: `items_host` (host memory, used for validation and
  * to size the launch) and `items_dev` (an identical copy in device memory,
- * read by the kernel).  `variant` is the number of H slices per lane of the
- * kernel configuration (1, 2 or 4 for either operator); see
- * vp_default_variant.  `grid` is the number of persistent workgroups
+ * read by the kernel).  `variant` selects the kernel configuration (1, 2 or 4
+ * slices per lane for either operator; 32, 64 or 128 slices per block for
+ * AT); see vp_default_variant.  `grid` is the number of persistent workgroups
  * (<= 0: one per work item).  Results do not depend on the grid or on the
  * other instances in the batch.  Because the launch reads only device
  * memory, it can be captured into a HIP graph and replayed.
@@ -210,12 +210,16 @@ typedef struct vp_transpose_item {
 VP_API int vp_transpose_batched(const vp_transpose_item *items_host,
                                 const vp_transpose_item *items_dev, int n, int grid, void *stream);
 
-/* Default kernel variant (slices per lane) of op 0 (A) / 1 (AT) for H slices,
- * the grouping key of batched launches.  vp_forward / vp_backward use it too,
- * except that 2 may drop to 1 for small and mid-size launches (fewer
- * workgroups than compute units, or fewer rounds of resident workgroups);
- * variants 1 and 2 give identical results, 4 adds the same terms in another
- * order (agreement to float32 rounding). */
+/* Default kernel variant of op 0 (A) / 1 (AT) for H slices, the grouping key
+ * of batched launches.  Variants 1, 2, 4: 1, 2 or 4 consecutive slices per
+ * lane (blocks of 64, 128, 256 slices); AT also has 32, 64, 128: blocks of
+ * that many slices at 4 slices per lane, the wave's lanes in rows of 8, 16,
+ * 32 that hold one voxel each (its default below 129 slices).  vp_forward /
+ * vp_backward use the default too, except that A's 2 may drop to 1 for
+ * launches with fewer workgroups than compute units.  Variants 1 and 2 give
+ * identical results; 4, 32, 64 and 128 identical results among themselves;
+ * the two families add the same terms in another order (agreement to
+ * float32 rounding). */
 VP_API int vp_default_variant(int op, int64_t H);
 
 /* Number of work items (workgroups) one instance contributes to a batched

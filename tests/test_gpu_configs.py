@@ -1,15 +1,19 @@
 """Launch configurations of the single-instance operators.
 
-A single launch picks its kernel configuration from the volume and the GPU:
-1 or 2 slices per lane, and for volumes whose grid would leave CUs idle the
-small backprojection tiles (16 x 8) and, up to 128 voxels per side, the
-narrow forward strips (16 pixels).  None of these choices changes the order
-in which an output adds its terms, so every configuration with 1 or 2 slices
-per lane must give the plain operator's result bit for bit.  The batched
-launches with an explicit variant use the default configurations (28-pixel
-strips; 16 x 16 and 16 x 24 tiles), so comparing them with the single launch
-checks exactly that.  (4 slices per lane uses 22-pixel strips and is covered
-by the tolerance tests.)
+A single launch picks its kernel configuration from the volume and the GPU.
+A: 1 or 2 slices per lane, and up to 128 voxels per side, for volumes whose
+grid would leave CUs idle, the narrow forward strips (16 pixels).  None of
+these choices changes the order in which an output adds its terms, so every
+A configuration with 1 or 2 slices per lane must give the plain operator's
+result bit for bit.  The batched launches with an explicit variant use the
+default configurations (28-pixel strips), so comparing them with the single
+launch checks exactly that.  (4 slices per lane uses 22-pixel strips and is
+covered by the tolerance tests.)
+AT: 4 slices per lane, 256 slices per block above 128 slices, below that
+rows of 8, 16 or 32 lanes per voxel (blocks of 32, 64 or 128 slices); all of
+them add each voxel's terms in the same order, so the single launch equals
+the batched launches of variants 4, 32, 64 and 128 bit for bit, and the 1-
+and 2-slice variants (another order) agree with it to float32 rounding.
 """
 import numpy as np
 import pytest
@@ -39,7 +43,7 @@ def _inputs(shape, seed):
 
 
 @pytest.mark.parametrize("shape", SHAPES)
-def test_single_launch_equals_every_1_2_slice_configuration(shape):
+def test_single_launch_equals_every_configuration_of_its_family(shape):
     SY, SX, H, A, PX = shape
     x, y, ang = _inputs(shape, 17)
     print(f"shape {shape}", flush=True)
@@ -49,9 +53,15 @@ def test_single_launch_equals_every_1_2_slice_configuration(shape):
         yv = torch.empty_like(ya)
         batch.BatchPlan("A", [x], [yv], [ang], accumulate=False, variant=v).launch()
         assert torch.equal(yv, ya), ("A", v)
+    outs = {}
+    for v in batch.BACKWARD_VARIANTS:
         xv = torch.empty_like(xb)
         batch.BatchPlan("AT", [y], [xv], [ang], accumulate=False, variant=v).launch()
-        assert torch.equal(xv, xb), ("AT", v)
+        outs[v] = xv
+    for v in (4, 32, 64, 128):
+        assert torch.equal(outs[v], xb), ("AT", v)
+    assert torch.equal(outs[1], outs[2])
+    assert float((outs[1] - xb).norm() / xb.norm()) <= 1e-6
 
 
 @pytest.mark.parametrize("shape", SHAPES[:3])

@@ -34,7 +34,7 @@ import torch
 from . import _lib, ops, recon
 
 FORWARD_VARIANTS = (1, 2, 4)
-BACKWARD_VARIANTS = (1, 2, 4)
+BACKWARD_VARIANTS = (1, 2, 4, 32, 64, 128)
 
 
 class _TItem(ctypes.Structure):

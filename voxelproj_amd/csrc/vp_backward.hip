@@ -635,6 +635,279 @@ __device__ __forceinline__ void backproject4_block(const OpArgs& g, const int lo
     }
 }
 
+// Small H: 4 slices per lane, RW lanes per voxel.  A wave's 64 lanes form
+// R = 64 / RW rows; row r holds voxel r of the wave's current R voxels, lane
+// q of a row slices hb + 4q .. hb + 4q + 3, so a block covers 4 * RW slices
+// (32, 64 or 128) with no idle lanes where H fills it.  Everything else is
+// the 4-slice kernel's: 8 x 8 tiles, footprints per group of 8 angles
+// anchored at their first nonzero pixel, two taps always and the third when
+// any row of the item needs it, windows staged by global -> LDS DMA into a
+// double buffer, sums in registers per group of 8 angles, then across
+// groups -- the same terms in the same order per voxel, so the results equal
+// the 4-slice kernel's bit for bit.  The tap weights and window offset of
+// each row's voxel-angle come from a per-wave LDS record (one ds_read_b128
+// per lane at a constant offset from the lane's row), so the FMAs take VGPR
+// operands and no v_readlane is needed.
+template <int TX, int TY, int NW, int AC, int WIN, int RW>
+__host__ __device__ inline void backward_rows_grid(int64_t SY, int64_t SX, int64_t H, int& nbx, int& nby,
+                                                   int& nbh)
+{
+    nbx = (int)((SX + TX - 1) / TX);
+    nby = (int)((SY + TY - 1) / TY);
+    nbh = (int)((H + 4 * RW - 1) / (4 * RW));
+}
+
+template <int TX, int TY, int NW, int AC, int WIN, int TAB, int RW>
+__device__ __forceinline__ void backproject_rows_block(const OpArgs& g, const int logical)
+{
+    constexpr int R = 64 / RW;             // voxel rows per wave
+    constexpr int ROW = 4 * RW;            // slices per block = floats per staged pixel row
+    constexpr int VPW = TX * TY / NW;      // voxels per wave
+    constexpr int VPL = VPW / R;           // voxels per lane
+    constexpr int APL = 64 / VPW;          // angles per footprint group
+    constexpr int NSUB = APL / AC;         // window chunks per group
+    constexpr int STAGE = AC * WIN * ROW;  // floats per window buffer
+    constexpr int NI = AC * VPL;           // tap items per chunk (R voxel-angles each)
+    static_assert(RW == 8 || RW == 16 || RW == 32, "rows of 8, 16 or 32 lanes");
+    static_assert(VPW % R == 0 && VPW * APL == 64 && APL % AC == 0 && NSUB % 2 == 0, "lane layout");
+    static_assert(WIN % R == 0, "a 16-byte DMA instruction stages R whole pixel rows of one angle");
+    static_assert((WIN * ROW) % 64 == 0, "a dword DMA instruction stays within one angle's rows");
+    // Window coverage as in the 4-slice kernel: the rows read are at most
+    // r + 1 - rlo <= floor(L) + 4 <= WIN - 1 (rlo = min r - 2 over the tile).
+    static_assert(isqrt((TX - 1) * (TX - 1) + (TY - 1) * (TY - 1)) + 5 <= WIN,
+                  "sinogram window too small for the tile");
+    __shared__ float4 s_wa[STAGE / 4];
+    __shared__ float4 s_wb[STAGE / 4];
+    constexpr int kTabAngles = TAB;
+    static_assert(kTabAngles % APL == 0 && kTabAngles <= 64 * NW, "angle table range");
+    __shared__ AngleK s_k[2][kTabAngles];
+    __shared__ int s_rlo[2][kTabAngles];
+    // per wave, lane = voxel-angle of the current group: (wl, wr, w3, window offset)
+    __shared__ float4 s_wt[NW][64];
+
+    const float* __restrict__ y = g.in;
+    float* __restrict__ x = g.out;
+    const float* __restrict__ angles = g.angles;
+    const int A = g.A, SY = g.SY, SX = g.SX, H = g.H, PX = g.PX;
+    const double scale = g.scale, inv_scale = g.inv_scale;
+    int nbx, nby, nbh_;
+    backward_rows_grid<TX, TY, NW, AC, WIN, RW>(SY, SX, H, nbx, nby, nbh_);
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = uni(tid >> 6);
+    const int bx = logical % nbx;
+    const int by = (logical / nbx) % nby;
+    const int bhc = logical / (nbx * nby);
+    const int tx0 = bx * TX, ty0 = by * TY;
+    const int hb = bhc * ROW;
+    const float xc = 0.5f * (float)(SX - 1);
+    const float yc = 0.5f * (float)(SY - 1);
+    const bool vec4 = (H & 3) == 0;
+    const int vr = lane / RW;  // this lane's voxel row
+    const int q4 = (lane % RW) * 4;  // its first slice, relative to hb
+
+    // footprint lane layout: voxel lv of this wave, angle la of the group
+    const int lv = lane % VPW;
+    const int la = lane / VPW;
+    const int vi = wave * VPW + lv;
+    const float X = (float)(tx0 + vi % TX) - xc;
+    const float Y = (float)(ty0 + vi / TX) - yc;
+
+    auto tables = [&](int a0, int tb) {
+        if (tid < kTabAngles && a0 + tid < A) {
+            const int a = a0 + tid;
+            const AngleK k = make_angle(angles[a], g.du ? g.du[a] : 0.0f, 0.5 * (double)(PX - 1), scale, inv_scale);
+            s_k[tb][tid] = k;
+            const float xa = (float)tx0 - xc;
+            const float xb = (float)(min(tx0 + TX, SX) - 1) - xc;
+            const float ya = (float)ty0 - yc;
+            const float yb = (float)(min(ty0 + TY, SY) - 1) - yc;
+            int r = voxel_r(xa, ya, k);
+            r = min(r, voxel_r(xb, ya, k));
+            r = min(r, voxel_r(xa, yb, k));
+            r = min(r, voxel_r(xb, yb, k));
+            s_rlo[tb][tid] = r - 2;
+        }
+    };
+    auto tab_of = [&](int a) { return (a / kTabAngles) & 1; };
+    auto ent_of = [&](int a) { return a % kTabAngles; };
+    // DMA of chunk `chunk`'s windows (pixel rows rlo .. rlo + WIN - 1 of each
+    // of its AC angles, slices hb .. hb + ROW - 1) into dst.  Each angle's
+    // rows are read through a buffer descriptor spanning exactly that angle's
+    // sinogram rows, so rows off the detector (and past the last angle) read
+    // zeros; their taps carry zero weight.  Lanes past H read the next pixel
+    // row's slices (or zeros) and are never stored.
+    auto fetch = [&](int chunk, float4* dst) __attribute__((always_inline)) {
+        const uint32_t range = (uint32_t)PX * (uint32_t)H * 4u;
+        if (vec4) {
+            constexpr int NINS = AC * WIN / R;  // 16-byte instructions: R pixel rows each
+            for (int ins = wave; ins < NINS; ins += NW) {
+                const int row0 = ins * R;          // first staged row: a * WIN + k0
+                const int a = row0 / WIN, k0 = row0 % WIN;
+                const int ag = min(chunk * AC + a, A - 1);
+                const int rlo = s_rlo[tab_of(ag)][ent_of(ag)];
+                const int pix = rlo + k0 + vr;
+                const uint32_t voff = (uint32_t)((pix * H + hb + q4) * 4);
+                dma_to_lds<16>(y + (int64_t)ag * PX * H, range, voff,
+                               (__attribute__((address_space(3))) void*)(dst + row0 * (ROW / 4)));
+            }
+        } else {
+            constexpr int NINS = AC * WIN * ROW / 64;  // dword instructions: 64 floats each
+            for (int ins = wave; ins < NINS; ins += NW) {
+                const int f = ins * 64 + lane;      // float of the chunk's windows
+                const int row = f / ROW, col = f % ROW;
+                const int a = (ins * 64) / (WIN * ROW), k = row % WIN;
+                const int ag = min(chunk * AC + a, A - 1);
+                const int rlo = s_rlo[tab_of(ag)][ent_of(ag)];
+                const uint32_t voff = (uint32_t)(((rlo + k) * H + hb + col) * 4);
+                dma_to_lds<4>(y + (int64_t)ag * PX * H, range, voff,
+                              (__attribute__((address_space(3))) void*)(reinterpret_cast<float*>(dst) + ins * 64));
+            }
+        }
+    };
+
+    float acc[VPL][4];
+#pragma unroll
+    for (int v = 0; v < VPL; ++v)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[v][j] = 0.0f;
+
+    const int nch = (A + AC - 1) / AC;
+    const int ngroups = (A + APL - 1) / APL;
+    tables(0, 0);
+    __syncthreads();
+    fetch(0, s_wa);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+
+    const float4* wt_row = &s_wt[wave][vr];  // this lane's row: entry L of the table at wt_row[L - vr]
+    for (int gi = 0; gi < ngroups; ++gi) {
+        const int a0 = gi * APL;
+        const int tb = tab_of(a0);
+        if ((a0 + APL) % kTabAngles == 0 && a0 + APL < A) {
+            tables(a0 + APL, tab_of(a0 + APL));
+            __syncthreads();
+        }
+        // footprint of (voxel lv, angle a0 + la); the group lies in one table range
+        uint64_t has3;  // bit L: entry L has a third tap
+        {
+            const int al = a0 + la;
+            const bool aval = al < A;
+            const int ae = ent_of(min(al, A - 1));
+            const AngleK k = s_k[tb][ae];
+            const Taps2 t = anchor_left(voxel_taps(X, Y, k));
+            const float wl = (aval && t.l >= 0 && t.l < PX) ? t.wl : 0.0f;
+            const float wr = (aval && t.l + 1 >= 0 && t.l + 1 < PX) ? t.wr : 0.0f;
+            const float w3 = (aval && t.l + 2 >= 0 && t.l + 2 < PX) ? t.w3 : 0.0f;
+            int off = t.l - s_rlo[tb][ae];
+            off = min(max(off, 0), w3 != 0.0f ? WIN - 3 : WIN - 2);
+            const int foff = ((la % AC) * WIN + off) * ROW;  // float offset of pixel l in its buffer
+            // (after the previous group's reads of s_wt: one wave's LDS accesses are performed in order)
+            s_wt[wave][lane] = make_float4(wl, wr, w3, __builtin_bit_cast(float, foff));
+            has3 = __ballot(w3 != 0.0f);
+        }
+
+        float part[VPL][4];
+#pragma unroll
+        for (int v = 0; v < VPL; ++v)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) part[v][j] = 0.0f;
+
+#pragma unroll
+        for (int sub = 0; sub < NSUB; ++sub) {
+            const int c = gi * NSUB + sub;
+            if (c >= nch) break;  // block-uniform
+            if (c + 1 < nch) fetch(c + 1, (sub & 1) ? s_wa : s_wb);
+            const float* sw = reinterpret_cast<const float*>((sub & 1) ? s_wb : s_wa) + q4;
+            // item i: angle sub * AC + i / VPL of the group, voxels (i % VPL) * R + row
+            auto entry = [&](int i) { return wt_row[(sub * AC + i / VPL) * VPW + (i % VPL) * R]; };
+            auto taps = [&](const float4& e, float4& t0, float4& t1) {
+                const float* pr = sw + __builtin_bit_cast(int, e.w);
+                t0 = *reinterpret_cast<const float4*>(pr);
+                t1 = *reinterpret_cast<const float4*>(pr + ROW);
+            };
+            float4 e_n = entry(0), t0_n, t1_n;
+            taps(e_n, t0_n, t1_n);
+            float4 e_n2 = NI > 1 ? entry(1) : e_n;
+#pragma unroll
+            for (int i = 0; i < NI; ++i) {
+                const int v = i % VPL;
+                const float4 e = e_n;
+                const float4 t0 = t0_n, t1 = t1_n;
+                if (i + 1 < NI) {
+                    e_n = e_n2;
+                    taps(e_n, t0_n, t1_n);
+                    if (i + 2 < NI) e_n2 = entry(i + 2);
+                }
+                part[v][0] = fmaf(e.x, t0.x, part[v][0]);
+                part[v][1] = fmaf(e.x, t0.y, part[v][1]);
+                part[v][2] = fmaf(e.x, t0.z, part[v][2]);
+                part[v][3] = fmaf(e.x, t0.w, part[v][3]);
+                part[v][0] = fmaf(e.y, t1.x, part[v][0]);
+                part[v][1] = fmaf(e.y, t1.y, part[v][1]);
+                part[v][2] = fmaf(e.y, t1.z, part[v][2]);
+                part[v][3] = fmaf(e.y, t1.w, part[v][3]);
+                // the third tap when any row's voxel-angle of the item has one
+                const int L0 = (sub * AC + i / VPL) * VPW + v * R;
+                if ((has3 >> L0) & ((1ull << R) - 1)) {
+                    const float4 t2 = *reinterpret_cast<const float4*>(sw + __builtin_bit_cast(int, e.w) + 2 * ROW);
+                    part[v][0] = fmaf(e.z, t2.x, part[v][0]);
+                    part[v][1] = fmaf(e.z, t2.y, part[v][1]);
+                    part[v][2] = fmaf(e.z, t2.z, part[v][2]);
+                    part[v][3] = fmaf(e.z, t2.w, part[v][3]);
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA into the other buffer landed
+            __syncthreads();
+        }
+#pragma unroll
+        for (int v = 0; v < VPL; ++v)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[v][j] += part[v][j];
+    }
+
+    // Write-out: old values first (clamped addresses), then the stores;
+    // 16-byte accesses where the lane's 4 slices exist and H is a multiple of 4.
+    const int hh0 = hb + q4;
+    const bool full = vec4 && hh0 + 4 <= H;
+    float4 old[VPL];
+    int64_t base[VPL];
+    bool vin[VPL];
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) {
+        const int wv = wave * VPW + v * R + vr;
+        const int ix = tx0 + wv % TX;
+        const int iy = ty0 + wv / TX;
+        vin[v] = ix < SX && iy < SY;
+        base[v] = vin[v] ? ((int64_t)iy * SX + ix) * H : 0;
+    }
+    if (full) {
+#pragma unroll
+        for (int v = 0; v < VPL; ++v)
+            old[v] = g.accumulate ? *reinterpret_cast<const float4*>(x + base[v] + hh0)
+                                  : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+            if (vin[v])
+                *reinterpret_cast<float4*>(x + base[v] + hh0) =
+                    make_float4(old[v].x + acc[v][0], old[v].y + acc[v][1], old[v].z + acc[v][2],
+                                old[v].w + acc[v][3]);
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+            if (vin[v]) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int hh = hh0 + j;
+                    if (hh < H) x[base[v] + hh] = (g.accumulate ? x[base[v] + hh] : 0.0f) + acc[v][j];
+                }
+            }
+        }
+    }
+}
+
 // Block order: consecutive workgroups (round-robin over the 8 XCDs) are
 // neighbouring tiles of one slice chunk, so concurrently running tiles read
 // overlapping sinogram rows, served from the shared Infinity Cache.
@@ -737,6 +1010,70 @@ __global__ __launch_bounds__((64 * NW), 4) void backproject4_batched_kernel(
     }
 }
 
+template <int TX, int TY, int NW, int AC, int WIN, int TAB, int RW>
+__global__ __launch_bounds__((64 * NW), 4) void backproject_rows_z0_kernel(OpArgs g)
+{
+    backproject_rows_block<TX, TY, NW, AC, WIN, TAB, RW>(g, xcd_contiguous(blockIdx.x, gridDim.x));
+}
+
+template <int TX, int TY, int NW, int AC, int WIN, int TAB, int RW>
+__global__ __launch_bounds__((64 * NW), 4) void backproject_rows_batched_kernel(
+    const BatchItem* __restrict__ items, int n, int64_t total)
+{
+    for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
+        int i = 0;
+        int64_t base = 0;
+        for (; i < n - 1; ++i) {
+            int nbx, nby, nbh;
+            backward_rows_grid<TX, TY, NW, AC, WIN, RW>(items[i].SY, items[i].SX, items[i].H, nbx, nby, nbh);
+            const int64_t nbl = (int64_t)nbx * nby * nbh;
+            if (w < base + nbl) break;
+            base += nbl;
+        }
+        backproject_rows_block<TX, TY, NW, AC, WIN, TAB, RW>(item_args(items[i]), (int)(w - base));
+        __syncthreads();
+    }
+}
+
+// Small-H configurations (block of 4 * RW slices): the 4-slice kernel's
+// 8 x 8 tiles, 8 waves, chunks of 2 angles, footprint groups of 8 angles;
+// windows of 16 pixels (a multiple of the R rows one DMA instruction stages)
+// for rows of 8 and 16 lanes, 14 for rows of 32.
+template <int RW>
+struct BwdRowsCfg {
+    static constexpr int TX = 8, TY = 8, NW = 8, AC = 2, WIN = (RW == 32 ? 14 : 16), TAB = 64;
+};
+
+template <int RW>
+hipError_t backward_rows_single(const OpArgs& g, hipStream_t s)
+{
+    using C = BwdRowsCfg<RW>;
+    int nbx, nby, nbh;
+    backward_rows_grid<C::TX, C::TY, C::NW, C::AC, C::WIN, RW>(g.SY, g.SX, g.H, nbx, nby, nbh);
+    const int64_t nb = (int64_t)nbx * nby * nbh;
+    if (nb > 0x7fffffff) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((backproject_rows_z0_kernel<C::TX, C::TY, C::NW, C::AC, C::WIN, C::TAB, RW>),
+                       dim3((unsigned)nb), dim3(64 * C::NW), 0, s, g);
+    return hipGetLastError();
+}
+
+template <int RW>
+hipError_t backward_rows_batched(const BatchItem* host, const BatchItem* dev, int n, int grid, hipStream_t s)
+{
+    using C = BwdRowsCfg<RW>;
+    int64_t total = 0;
+    for (int i = 0; i < n; ++i) {
+        int nbx, nby, nbh;
+        backward_rows_grid<C::TX, C::TY, C::NW, C::AC, C::WIN, RW>(host[i].SY, host[i].SX, host[i].H, nbx, nby, nbh);
+        total += (int64_t)nbx * nby * nbh;
+    }
+    if (total == 0) return hipSuccess;
+    const int64_t gsz = grid > 0 ? std::min<int64_t>(grid, total) : total;
+    hipLaunchKernelGGL((backproject_rows_batched_kernel<C::TX, C::TY, C::NW, C::AC, C::WIN, C::TAB, RW>),
+                       dim3((unsigned)gsz), dim3(64 * C::NW), 0, s, dev, n, total);
+    return hipGetLastError();
+}
+
 hipError_t backward4_single(const OpArgs& g, hipStream_t s)
 {
     using C = Bwd4Cfg;
@@ -824,6 +1161,7 @@ hipError_t backward_batched(const BatchItem* host, const BatchItem* dev, int n, 
 
 int default_variant(int op, int64_t H)
 {
+    if (op == 1 && H <= 128) return H <= 32 ? 32 : (H <= 64 ? 64 : 128);  // AT: rows of 8, 16, 32 lanes
     return H > 128 ? 4 : (H > 64 ? 2 : 1);
 }
 
@@ -845,7 +1183,10 @@ int device_cus()
 int single_variant(int op, int A, int64_t SY, int64_t SX, int64_t H, int64_t PX)
 {
     const int v = default_variant(op, H);
-    if (v != 2) return v;  // 4 slices per lane is kept: it adds terms in its own order
+    // AT's configurations (4 slices per lane, 256 slices per block or rows
+    // of 8, 16, 32 lanes per voxel below 129 slices) all add their terms in
+    // the same order, and the row configurations fill any volume's lanes
+    if (op == 1 || v != 2) return v;  // 4 slices per lane is kept: it adds terms in its own order
     const int cus = device_cus();
     if (op == 0) return forward_blocks(2, A, H, PX) < cus ? 1 : 2;
     // AT: with fewer 2-slice workgroups than CUs, 1 slice per lane.  Else
@@ -881,6 +1222,9 @@ hipError_t launch_backward_z0(const float* y, float* x, const float* angles, con
     a.du = g.du;
     const int v = single_variant(1, g.A, g.SY, g.SX, g.H, g.PX);
     if (v == 4) return backward4_single(a, s);
+    if (v == 32) return backward_rows_single<8>(a, s);
+    if (v == 64) return backward_rows_single<16>(a, s);
+    if (v == 128) return backward_rows_single<32>(a, s);
     if (v == 2) return backward_single<2>(a, s);
     if (backward_blocks(1, g.SY, g.SX, g.H) < device_cus()) return backward_launch<BwdSmall, 1>(a, s);
     return backward_single<1>(a, s);
@@ -893,6 +1237,9 @@ hipError_t launch_backward_batched(int variant, const BatchItem* items_host,
         case 1: return backward_batched<1>(items_host, items_dev, n, grid, s);
         case 2: return backward_batched<2>(items_host, items_dev, n, grid, s);
         case 4: return backward4_batched(items_host, items_dev, n, grid, s);
+        case 32: return backward_rows_batched<8>(items_host, items_dev, n, grid, s);
+        case 64: return backward_rows_batched<16>(items_host, items_dev, n, grid, s);
+        case 128: return backward_rows_batched<32>(items_host, items_dev, n, grid, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -904,6 +1251,9 @@ int64_t backward_blocks(int variant, int64_t SY, int64_t SX, int64_t H)
         case 1: backward_grid<BwdCfg<1>::TX, BwdCfg<1>::TY, 1>(SY, SX, H, nbx, nby, nbh); break;
         case 2: backward_grid<BwdCfg<2>::TX, BwdCfg<2>::TY, 2>(SY, SX, H, nbx, nby, nbh); break;
         case 4: backward4_grid<Bwd4Cfg::TX, Bwd4Cfg::TY, Bwd4Cfg::NW, Bwd4Cfg::AC, Bwd4Cfg::WIN>(SY, SX, H, nbx, nby, nbh); break;
+        case 32: backward_rows_grid<8, 8, 8, 2, 16, 8>(SY, SX, H, nbx, nby, nbh); break;
+        case 64: backward_rows_grid<8, 8, 8, 2, 16, 16>(SY, SX, H, nbx, nby, nbh); break;
+        case 128: backward_rows_grid<8, 8, 8, 2, 14, 32>(SY, SX, H, nbx, nby, nbh); break;
         default: return -1;
     }
     return (int64_t)nbx * nby * nbh;

@@ -33,14 +33,16 @@ struct BatchItem {
     int32_t accumulate;
 };
 
-// Slices per lane of the kernel variants (the autotuner's search space).
+// Kernel variants (the autotuner's search space): 1, 2, 4 = slices per lane
+// (blocks of 64, 128, 256 slices); for AT also 32, 64, 128 = blocks of that
+// many slices at 4 slices per lane, a wave's lanes split into rows of 8, 16,
+// 32 lanes that hold one voxel each.
 constexpr int kForwardVariants[] = {1, 2, 4};
-constexpr int kBackwardVariants[] = {1, 2, 4};
+constexpr int kBackwardVariants[] = {1, 2, 4, 32, 64, 128};
 int default_variant(int op, int64_t H);  // op 0 = A, 1 = AT
 // Variant of a single (non-batched) launch: the H-based default, except that
 // 2 slices per lane drops to 1 for A when its grid has fewer workgroups than
-// the device has CUs, and for AT when an estimate of the rounds of resident
-// workgroups favours it (small and mid-size volumes).
+// the device has CUs.
 int single_variant(int op, int A, int64_t SY, int64_t SX, int64_t H, int64_t PX);
 int device_cus();  // compute units of the current device (cached per device)
 // Work items (blocks) of one instance under a variant; -1 for an unknown variant.
