@@ -983,6 +983,17 @@ __device__ __forceinline__ int xcd_contiguous(int bid, int nb)
     return xcd < r ? xcd * (q + 1) + k : r * (q + 1) + (xcd - r) * q + k;
 }
 
+// Work item of persistent workgroup w's current round in a batched launch:
+// the items of each round (gridDim.x of them, fewer in the last) are handed
+// out in the XCD-contiguous order above, so neighbouring tiles of an
+// instance share an L2 as in a single launch.  A bijection on each round.
+__device__ __forceinline__ int64_t xcd_round_item(int64_t w, int64_t total)
+{
+    const int64_t round0 = w - w % gridDim.x;
+    const int nb = (int)min<int64_t>(gridDim.x, total - round0);
+    return round0 + xcd_contiguous((int)(w - round0), nb);
+}
+
 // Register budget for 4 waves per SIMD (128 VGPRs): two 8-wave workgroups
 // per CU, which hide each other's chunk barriers.
 template <int TX, int TY, int NW, int AC, int WIN, int TAB>
@@ -995,7 +1006,8 @@ template <int TX, int TY, int NW, int AC, int WIN, int TAB>
 __global__ __launch_bounds__((64 * NW), 4) void backproject4_batched_kernel(
     const BatchItem* __restrict__ items, int n, int64_t total)
 {
-    for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
+    for (int64_t w0 = blockIdx.x; w0 < total; w0 += gridDim.x) {
+        const int64_t w = xcd_round_item(w0, total);
         int i = 0;
         int64_t base = 0;
         for (; i < n - 1; ++i) {
@@ -1020,7 +1032,8 @@ template <int TX, int TY, int NW, int AC, int WIN, int TAB, int RW>
 __global__ __launch_bounds__((64 * NW), 4) void backproject_rows_batched_kernel(
     const BatchItem* __restrict__ items, int n, int64_t total)
 {
-    for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
+    for (int64_t w0 = blockIdx.x; w0 < total; w0 += gridDim.x) {
+        const int64_t w = xcd_round_item(w0, total);
         int i = 0;
         int64_t base = 0;
         for (; i < n - 1; ++i) {
