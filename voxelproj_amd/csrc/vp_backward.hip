@@ -515,11 +515,11 @@ __device__ __forceinline__ void backproject4_block(const OpArgs& g, const int lo
             if (lane < NSUB * RPW && row < AC * WIN) rows_l = dma_row(gi * NSUB + sub + 1, row);
         }
 
+        // The group's first item of voxel v (sub 0, i = v; chunk gi * NSUB
+        // always holds an angle) sets part[v] to its first product instead of
+        // adding it to zeros: the sums differ at most in the sign of a zero,
+        // which acc += part removes, so every result is unchanged.
         float part[VPW][HL];
-#pragma unroll
-        for (int v = 0; v < VPW; ++v)
-#pragma unroll
-            for (int j = 0; j < HL; ++j) part[v][j] = 0.0f;
 
 #pragma unroll
         for (int sub = 0; sub < NSUB; ++sub) {
@@ -567,10 +567,17 @@ __device__ __forceinline__ void backproject4_block(const OpArgs& g, const int lo
                 }
                 const float c0 = cw.x;
                 const float c1 = cw.y;
-                part[v][0] = fmaf(c0, cur.t[0].x, part[v][0]);
-                part[v][1] = fmaf(c0, cur.t[0].y, part[v][1]);
-                part[v][2] = fmaf(c0, cur.t[0].z, part[v][2]);
-                part[v][3] = fmaf(c0, cur.t[0].w, part[v][3]);
+                if (sub == 0 && i < VPW) {  // compile-time
+                    part[v][0] = c0 * cur.t[0].x;
+                    part[v][1] = c0 * cur.t[0].y;
+                    part[v][2] = c0 * cur.t[0].z;
+                    part[v][3] = c0 * cur.t[0].w;
+                } else {
+                    part[v][0] = fmaf(c0, cur.t[0].x, part[v][0]);
+                    part[v][1] = fmaf(c0, cur.t[0].y, part[v][1]);
+                    part[v][2] = fmaf(c0, cur.t[0].z, part[v][2]);
+                    part[v][3] = fmaf(c0, cur.t[0].w, part[v][3]);
+                }
                 part[v][0] = fmaf(c1, cur.t[1].x, part[v][0]);
                 part[v][1] = fmaf(c1, cur.t[1].y, part[v][1]);
                 part[v][2] = fmaf(c1, cur.t[1].z, part[v][2]);
@@ -808,11 +815,7 @@ __device__ __forceinline__ void backproject_rows_block(const OpArgs& g, const in
             has3 = __ballot(w3 != 0.0f);
         }
 
-        float part[VPL][4];
-#pragma unroll
-        for (int v = 0; v < VPL; ++v)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) part[v][j] = 0.0f;
+        float part[VPL][4];  // set by the group's first item of each voxel, as in the 4-slice kernel
 
 #pragma unroll
         for (int sub = 0; sub < NSUB; ++sub) {
@@ -840,10 +843,17 @@ __device__ __forceinline__ void backproject_rows_block(const OpArgs& g, const in
                     taps(e_n, t0_n, t1_n);
                     if (i + 2 < NI) e_n2 = entry(i + 2);
                 }
-                part[v][0] = fmaf(e.x, t0.x, part[v][0]);
-                part[v][1] = fmaf(e.x, t0.y, part[v][1]);
-                part[v][2] = fmaf(e.x, t0.z, part[v][2]);
-                part[v][3] = fmaf(e.x, t0.w, part[v][3]);
+                if (sub == 0 && i < VPL) {  // compile-time
+                    part[v][0] = e.x * t0.x;
+                    part[v][1] = e.x * t0.y;
+                    part[v][2] = e.x * t0.z;
+                    part[v][3] = e.x * t0.w;
+                } else {
+                    part[v][0] = fmaf(e.x, t0.x, part[v][0]);
+                    part[v][1] = fmaf(e.x, t0.y, part[v][1]);
+                    part[v][2] = fmaf(e.x, t0.z, part[v][2]);
+                    part[v][3] = fmaf(e.x, t0.w, part[v][3]);
+                }
                 part[v][0] = fmaf(e.y, t1.x, part[v][0]);
                 part[v][1] = fmaf(e.y, t1.y, part[v][1]);
                 part[v][2] = fmaf(e.y, t1.z, part[v][2]);
