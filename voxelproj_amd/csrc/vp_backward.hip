@@ -600,10 +600,57 @@ __device__ __forceinline__ void backproject4_block(const OpArgs& g, const int lo
             for (int j = 0; j < HL; ++j) acc[v][j] += part[v][j];
     }
 
+    const int hh0 = hb + lane * HL;
+    if (g.out_z2) {
+        // x in (H, SY, SX): the wave's voxels are one tile row of TX
+        // consecutive voxels, so per slice a lane adds its VPW sums to one
+        // run of consecutive floats: two 16-byte accesses when SX is a
+        // multiple of 4 and the row lies inside the volume (tx0 is a multiple
+        // of 8).  Old values first, then the stores.
+        static_assert(VPW == TX && VPW == 8, "one tile row of 8 voxels per wave");
+        const int iy = ty0 + wave;
+        if (iy < SY) {
+            const int64_t plane = (int64_t)SY * SX;
+            float* row = x + (int64_t)iy * SX + tx0;
+            if ((SX & 3) == 0 && tx0 + TX <= SX) {
+                float4 o[HL][2];
+#pragma unroll
+                for (int j = 0; j < HL; ++j) {
+                    const int64_t off = (int64_t)min(hh0 + j, H - 1) * plane;
+#pragma unroll
+                    for (int k = 0; k < 2; ++k)
+                        o[j][k] = g.accumulate ? *reinterpret_cast<const float4*>(row + off + 4 * k)
+                                               : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+#pragma unroll
+                for (int j = 0; j < HL; ++j) {
+                    if (hh0 + j < H) {
+                        const int64_t off = (int64_t)(hh0 + j) * plane;
+#pragma unroll
+                        for (int k = 0; k < 2; ++k)
+                            *reinterpret_cast<float4*>(row + off + 4 * k) =
+                                make_float4(o[j][k].x + acc[4 * k][j], o[j][k].y + acc[4 * k + 1][j],
+                                            o[j][k].z + acc[4 * k + 2][j], o[j][k].w + acc[4 * k + 3][j]);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < HL; ++j) {
+                    if (hh0 + j < H) {
+                        float* p = row + (int64_t)(hh0 + j) * plane;
+#pragma unroll
+                        for (int v = 0; v < VPW; ++v)
+                            if (tx0 + v < SX) p[v] = (g.accumulate ? p[v] : 0.0f) + acc[v][j];
+                    }
+                }
+            }
+        }
+        return;
+    }
+
     // Write-out: the old values of the wave's voxels are loaded first (clamped
     // addresses), then all stores; 16-byte accesses where the lane's 4 slices
     // exist and H is a multiple of 4.
-    const int hh0 = hb + lane * HL;
     const bool full = vec4 && hh0 + HL <= H;
     float4 old[VPW];
     int64_t base[VPW];
@@ -936,6 +983,7 @@ __device__ __forceinline__ OpArgs item_args(const BatchItem& it)
     g.H = it.H;
     g.PX = it.PX;
     g.accumulate = it.accumulate;
+    g.out_z2 = 0;
     const int bits = hi_bits(it.SY, it.SX, it.PX);
     g.du = nullptr;
     g.scale = ldexp(1.0, bits);
@@ -1227,8 +1275,9 @@ int single_variant(int op, int A, int64_t SY, int64_t SX, int64_t H, int64_t PX)
     return t1 < t2 ? 1 : 2;
 }
 
-hipError_t launch_backward_z0(const float* y, float* x, const float* angles, const Geom& g,
-                              int accumulate, hipStream_t s)
+namespace {
+OpArgs backward_args(const float* y, float* x, const float* angles, const Geom& g, int accumulate,
+                     int out_z2)
 {
     OpArgs a;
     a.in = y;
@@ -1240,9 +1289,28 @@ hipError_t launch_backward_z0(const float* y, float* x, const float* angles, con
     a.H = (int)g.H;
     a.PX = (int)g.PX;
     a.accumulate = accumulate;
+    a.out_z2 = out_z2;
     a.scale = ldexp(1.0, g.bits);
     a.inv_scale = ldexp(1.0, -g.bits);
     a.du = g.du;
+    return a;
+}
+}  // namespace
+
+bool backward_writes_z2(const Geom& g) { return single_variant(1, g.A, g.SY, g.SX, g.H, g.PX) == 4; }
+
+// Always the 4-slice kernel (also for a slab of 128 slices or fewer, where
+// single launches use the row kernels: their sums are bitwise the same).
+hipError_t launch_backward_to_z2(const float* y, float* x, const float* angles, const Geom& g,
+                                 hipStream_t s)
+{
+    return backward4_single(backward_args(y, x, angles, g, 1, 1), s);
+}
+
+hipError_t launch_backward_z0(const float* y, float* x, const float* angles, const Geom& g,
+                              int accumulate, hipStream_t s)
+{
+    const OpArgs a = backward_args(y, x, angles, g, accumulate, 0);
     const int v = single_variant(1, g.A, g.SY, g.SX, g.H, g.PX);
     if (v == 4) return backward4_single(a, s);
     if (v == 32) return backward_rows_single<8>(a, s);

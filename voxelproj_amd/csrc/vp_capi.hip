@@ -182,14 +182,17 @@ int64_t slab_min_elems(const vp::Geom& g) { return g.SY * g.SX + (int64_t)g.A * 
 // out (+)= op(in) in z_order 2 without translations, through H slabs of the
 // workspace (z_order 2 slabs are contiguous): the input slab is transposed
 // into slice-fastest order, the z_order 0 kernel runs on it, and its result
-// is transposed and added into the output slab.  Slab heights are multiples
-// of 256 slices (a whole slice block of every kernel variant) unless the
-// whole H fits or fewer than 256 slices do.
+// is transposed and added into the output slab -- except where the
+// backprojection's 4-slice kernel adds its sums straight into the z_order 2
+// volume (vp::backward_writes_z2), which needs no volume workspace.  Slab
+// heights are multiples of 256 slices (a whole slice block of every kernel
+// variant) unless the whole H fits or fewer than 256 slices do.
 hipError_t z2_by_slabs(int op, const float* in, float* out, const float* angles,
                        const vp::Geom& g, float* ws, int64_t ws_elems, hipStream_t s)
 {
     const int64_t nv = g.SY * g.SX, ns = (int64_t)g.A * g.PX;  // floats per slice
-    int64_t hs = ws_elems / (nv + ns);
+    const bool direct = op == 1 && vp::backward_writes_z2(g);
+    int64_t hs = ws_elems / (direct ? ns : nv + ns);
     if (hs >= g.H)
         hs = g.H;
     else if (hs >= 256)
@@ -204,6 +207,9 @@ hipError_t z2_by_slabs(int op, const float* in, float* out, const float* angles,
             e = vp::launch_transpose(in + h0 * nv, wx, gs.H, nv, 0, s);
             if (e == hipSuccess) e = vp::launch_forward_z0(wx, wy, angles, gs, 0, s);
             if (e == hipSuccess) e = vp::launch_transpose(wy, out + h0 * ns, ns, gs.H, 1, s);
+        } else if (direct) {
+            e = vp::launch_transpose(in + h0 * ns, ws, gs.H, ns, 0, s);
+            if (e == hipSuccess) e = vp::launch_backward_to_z2(ws, out + h0 * nv, angles, gs, s);
         } else {
             e = vp::launch_transpose(in + h0 * ns, wy, gs.H, ns, 0, s);
             if (e == hipSuccess) e = vp::launch_backward_z0(wy, wx, angles, gs, 0, s);
@@ -270,6 +276,10 @@ hipError_t backward_f32(int z_order, const float* y, float* x, const float* angl
     if (e == hipSuccess) e = vp::launch_vshift<float>(0, 1, yz, g0, g.A, g.PX, g.H, 0, g.H, 0, dv, 0, s);
     if (z_order == 0) {
         if (e == hipSuccess) e = vp::launch_backward_z0(g0, x, angles, g, 1, s);
+        return e;
+    }
+    if (vp::backward_writes_z2(g)) {
+        if (e == hipSuccess) e = vp::launch_backward_to_z2(g0, x, angles, g, s);
         return e;
     }
     if (e == hipSuccess) e = vp::launch_backward_z0(g0, x0, angles, g, 0, s);

@@ -612,6 +612,7 @@ __device__ __forceinline__ OpArgs item_args(const BatchItem& it)
     g.H = it.H;
     g.PX = it.PX;
     g.accumulate = it.accumulate;
+    g.out_z2 = 0;
     const int bits = hi_bits(it.SY, it.SX, it.PX);
     g.du = nullptr;
     g.scale = ldexp(1.0, bits);
@@ -820,6 +821,7 @@ hipError_t launch_forward_z0(const float* x, float* y, const float* angles, cons
     a.H = (int)g.H;
     a.PX = (int)g.PX;
     a.accumulate = accumulate;
+    a.out_z2 = 0;
     a.scale = ldexp(1.0, g.bits);
     a.inv_scale = ldexp(1.0, -g.bits);
     a.du = g.du;

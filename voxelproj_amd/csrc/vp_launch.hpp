@@ -22,6 +22,7 @@ struct OpArgs {
     int accumulate;  // 1: out += op(in), 0: out = op(in)
     double scale, inv_scale;  // 2^B and 2^-B of the cos/sin hi split
     const float* du;          // per-angle detector translation (pixels), or null
+    int out_z2;               // AT, 4 slices per lane: out is (H, SY, SX) instead of (SY, SX, H)
 };
 
 // One instance of a batched launch (layout shared with the C ABI's vp_batch_item).
@@ -59,6 +60,13 @@ hipError_t launch_backward_batched(int variant, const BatchItem* items_host,
 // x (+)= AT y in z_order 0 (accumulate != 0 adds, else overwrites).
 hipError_t launch_backward_z0(const float* y, float* x, const float* angles, const Geom& g,
                               int accumulate, hipStream_t s);
+// x += AT y with y in z_order 0 and x in z_order 2 (H, SY, SX), written
+// directly by the 4-slice kernel; backward_writes_z2 tells whether that is
+// the single launch's kernel for g (else the row kernels, with transposes,
+// are faster).
+bool backward_writes_z2(const Geom& g);
+hipError_t launch_backward_to_z2(const float* y, float* x, const float* angles, const Geom& g,
+                                 hipStream_t s);
 
 // y (+)= A x in z_order 0 (accumulate != 0 adds, else overwrites).
 hipError_t launch_forward_z0(const float* x, float* y, const float* angles, const Geom& g,
