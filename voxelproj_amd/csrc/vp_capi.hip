@@ -162,18 +162,21 @@ hipError_t cached_workspace(int64_t bytes, hipStream_t s, void** out)
     return hipSuccess;
 }
 
-// Workspace (elements of T) that a call needs: z_order 2 transposes the
-// volume and the sinogram (|x| + |y| at full H, or H slabs of a smaller
-// buffer); dv needs the untranslated sinogram (|y|, and with z_order 2 a
-// second one).  Float32 paths only use the z_order 0 kernels; FP64 has
-// native z_order 2 kernels and needs a workspace only for dv.
-int64_t workspace_elems(bool f64, int z_order, const vp::Geom& g, bool dv)
+// Workspace (elements of T) that a call of operator op (0 = A, 1 = AT, -1 =
+// the larger of the two) needs: z_order 2 transposes the volume and the
+// sinogram (|x| + |y| at full H, or H slabs of a smaller buffer), except that
+// AT's 4-slice kernel writes a z_order 2 volume itself (|y|); dv needs the
+// untranslated sinogram (|y|, and with z_order 2 a second one).  Float32
+// paths only use the z_order 0 kernels; FP64 has native z_order 2 kernels and
+// needs a workspace only for dv.
+int64_t workspace_elems(bool f64, int z_order, const vp::Geom& g, bool dv, int op)
 {
     const int64_t nx = g.SY * g.SX * g.H;
     const int64_t ny = (int64_t)g.A * g.PX * g.H;
     if (f64) return dv ? ny : 0;
-    if (!dv) return z_order == 2 ? nx + ny : 0;
-    return z_order == 2 ? nx + 2 * ny : ny;
+    const int64_t vol = (op == 1 && vp::backward_writes_z2(g)) ? 0 : nx;
+    if (!dv) return z_order == 2 ? vol + ny : 0;
+    return z_order == 2 ? vol + 2 * ny : ny;
 }
 
 // Minimum workspace of the slab path: one slice of volume and sinogram.
@@ -263,11 +266,15 @@ hipError_t backward_f32(int z_order, const float* y, float* x, const float* angl
         if (z_order == 0) return vp::launch_backward_z0(y, x, angles, g, 1, s);
         return z2_by_slabs(1, y, x, angles, g, ws, ws_elems, s);
     }
-    // dv: the adjoint H interpolation first, then AT of the untranslated sinogram
+    // dv: the adjoint H interpolation first, then AT of the untranslated
+    // sinogram.  Workspace: [the volume before its transpose (z_order 2, not
+    // when the kernel writes z_order 2 itself)] [transposed sinogram
+    // (z_order 2)] [interpolated sinogram].
     hipError_t e = hipSuccess;
+    const bool direct = z_order == 2 && vp::backward_writes_z2(g);
     const float* yz = y;  // sinogram in slice-fastest order
-    float* x0 = ws;       // z_order 2: the volume before its transpose
-    float* yb = z_order == 2 ? ws + nx : ws;
+    float* x0 = ws;
+    float* yb = (z_order == 2 && !direct) ? ws + nx : ws;
     if (z_order == 2) {
         e = vp::launch_transpose(y, yb, g.H, (int64_t)g.A * g.PX, 0, s);
         yz = yb;
@@ -278,7 +285,7 @@ hipError_t backward_f32(int z_order, const float* y, float* x, const float* angl
         if (e == hipSuccess) e = vp::launch_backward_z0(g0, x, angles, g, 1, s);
         return e;
     }
-    if (vp::backward_writes_z2(g)) {
+    if (direct) {
         if (e == hipSuccess) e = vp::launch_backward_to_z2(g0, x, angles, g, s);
         return e;
     }
@@ -313,7 +320,7 @@ hipError_t backward_f64(int z_order, const double* y, double* x, const float* an
 }
 
 template <typename T, typename Op>
-int run_op(Op op, const char* what, int z_order, const T* in, T* out, const float* angles,
+int run_op(Op op, int which, const char* what, int z_order, const T* in, T* out, const float* angles,
            int n_angles, int64_t SY, int64_t SX, int64_t H, int64_t PX, const float* du,
            const float* dv, double du_max, T* ws, int64_t ws_elems, void* stream)
 {
@@ -327,7 +334,7 @@ int run_op(Op op, const char* what, int z_order, const T* in, T* out, const floa
     if (ws_elems < 0 || (ws_elems > 0 && ws == nullptr))
         return fail(VP_ERR_ARG, "workspace: negative size or null pointer");
     const bool f64 = sizeof(T) == 8;
-    const int64_t need = workspace_elems(f64, z_order, g, dv != nullptr);
+    const int64_t need = workspace_elems(f64, z_order, g, dv != nullptr, which);
     hipStream_t s = (hipStream_t)stream;
     hipError_t e;
     if (need == 0) {
@@ -391,28 +398,28 @@ extern "C" {
 int vp_forward(int z_order, const float* x, float* y, const float* angles, int n_angles,
                int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
 {
-    return run_op<float>(forward_f32, "forward", z_order, x, y, angles, n_angles, SY, SX, H, PX,
+    return run_op<float>(forward_f32, 0, "forward", z_order, x, y, angles, n_angles, SY, SX, H, PX,
                          nullptr, nullptr, 0.0, nullptr, 0, stream);
 }
 
 int vp_backward(int z_order, const float* y, float* x, const float* angles, int n_angles,
                 int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
 {
-    return run_op<float>(backward_f32, "backward", z_order, y, x, angles, n_angles, SY, SX, H, PX,
+    return run_op<float>(backward_f32, 1, "backward", z_order, y, x, angles, n_angles, SY, SX, H, PX,
                          nullptr, nullptr, 0.0, nullptr, 0, stream);
 }
 
 int vp_forward_f64(int z_order, const double* x, double* y, const float* angles, int n_angles,
                    int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
 {
-    return run_op<double>(forward_f64, "forward f64", z_order, x, y, angles, n_angles, SY, SX, H,
+    return run_op<double>(forward_f64, 0, "forward f64", z_order, x, y, angles, n_angles, SY, SX, H,
                           PX, nullptr, nullptr, 0.0, nullptr, 0, stream);
 }
 
 int vp_backward_f64(int z_order, const double* y, double* x, const float* angles, int n_angles,
                     int64_t SY, int64_t SX, int64_t H, int64_t PX, void* stream)
 {
-    return run_op<double>(backward_f64, "backward f64", z_order, y, x, angles, n_angles, SY, SX, H,
+    return run_op<double>(backward_f64, 1, "backward f64", z_order, y, x, angles, n_angles, SY, SX, H,
                           PX, nullptr, nullptr, 0.0, nullptr, 0, stream);
 }
 
@@ -420,7 +427,7 @@ int vp_forward_geom(int z_order, const float* x, float* y, const float* angles, 
                     int64_t SY, int64_t SX, int64_t H, int64_t PX, const float* du,
                     const float* dv, double du_max, void* stream)
 {
-    return run_op<float>(forward_f32, "forward", z_order, x, y, angles, n_angles, SY, SX, H, PX,
+    return run_op<float>(forward_f32, 0, "forward", z_order, x, y, angles, n_angles, SY, SX, H, PX,
                          du, dv, du_max, nullptr, 0, stream);
 }
 
@@ -428,7 +435,7 @@ int vp_backward_geom(int z_order, const float* y, float* x, const float* angles,
                      int64_t SY, int64_t SX, int64_t H, int64_t PX, const float* du,
                      const float* dv, double du_max, void* stream)
 {
-    return run_op<float>(backward_f32, "backward", z_order, y, x, angles, n_angles, SY, SX, H, PX,
+    return run_op<float>(backward_f32, 1, "backward", z_order, y, x, angles, n_angles, SY, SX, H, PX,
                          du, dv, du_max, nullptr, 0, stream);
 }
 
@@ -436,7 +443,7 @@ int vp_forward_geom_f64(int z_order, const double* x, double* y, const float* an
                         int n_angles, int64_t SY, int64_t SX, int64_t H, int64_t PX,
                         const float* du, const float* dv, double du_max, void* stream)
 {
-    return run_op<double>(forward_f64, "forward f64", z_order, x, y, angles, n_angles, SY, SX, H,
+    return run_op<double>(forward_f64, 0, "forward f64", z_order, x, y, angles, n_angles, SY, SX, H,
                           PX, du, dv, du_max, nullptr, 0, stream);
 }
 
@@ -444,7 +451,7 @@ int vp_backward_geom_f64(int z_order, const double* y, double* x, const float* a
                          int n_angles, int64_t SY, int64_t SX, int64_t H, int64_t PX,
                          const float* du, const float* dv, double du_max, void* stream)
 {
-    return run_op<double>(backward_f64, "backward f64", z_order, y, x, angles, n_angles, SY, SX, H,
+    return run_op<double>(backward_f64, 1, "backward f64", z_order, y, x, angles, n_angles, SY, SX, H,
                           PX, du, dv, du_max, nullptr, 0, stream);
 }
 
@@ -452,7 +459,7 @@ int vp_forward_ws(int z_order, const float* x, float* y, const float* angles, in
                   int64_t SY, int64_t SX, int64_t H, int64_t PX, const float* du, const float* dv,
                   double du_max, float* workspace, int64_t workspace_elems, void* stream)
 {
-    return run_op<float>(forward_f32, "forward", z_order, x, y, angles, n_angles, SY, SX, H, PX,
+    return run_op<float>(forward_f32, 0, "forward", z_order, x, y, angles, n_angles, SY, SX, H, PX,
                          du, dv, du_max, workspace, workspace_elems, stream);
 }
 
@@ -460,7 +467,7 @@ int vp_backward_ws(int z_order, const float* y, float* x, const float* angles, i
                    int64_t SY, int64_t SX, int64_t H, int64_t PX, const float* du, const float* dv,
                    double du_max, float* workspace, int64_t workspace_elems, void* stream)
 {
-    return run_op<float>(backward_f32, "backward", z_order, y, x, angles, n_angles, SY, SX, H, PX,
+    return run_op<float>(backward_f32, 1, "backward", z_order, y, x, angles, n_angles, SY, SX, H, PX,
                          du, dv, du_max, workspace, workspace_elems, stream);
 }
 
@@ -475,7 +482,7 @@ int64_t vp_workspace_elems(int f64, int z_order, int n_angles, int64_t SY, int64
     g.H = H;
     g.PX = PX;
     g.A = n_angles;
-    return workspace_elems(f64 != 0, z_order, g, with_dv != 0);
+    return workspace_elems(f64 != 0, z_order, g, with_dv != 0, -1);
 }
 
 int vp_release_workspaces(void)
