@@ -118,8 +118,10 @@ VP_API int vp_backward_ws(int z_order, const float *y, float *x, const float *an
                           void *stream);
 
 /* Workspace elements (floats, or doubles when f64 != 0) that one call of the
- * given layout and extents needs without H slabs; 0 when it needs none
- * (z_order 0 without dv); < 0 on invalid input. */
+ * given layout and extents needs without H slabs, for either operator (the
+ * larger need: vp_backward in z_order 2 above 128 slices needs no volume
+ * part, since its kernel writes the z_order 2 volume itself); 0 when it needs
+ * none (z_order 0 without dv); < 0 on invalid input. */
 VP_API int64_t vp_workspace_elems(int f64, int z_order, int n_angles, int64_t SY, int64_t SX,
                                   int64_t H, int64_t PX, int with_dv);
 
