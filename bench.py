@@ -268,6 +268,9 @@ def main():
     ap.add_argument("--profile", action="store_true",
                     help="short run for rocprofv3 counters: one 512 MiB copy (reference "
                          "traffic), then one A and one AT launch; prints no JSON")
+    ap.add_argument("--profile-z2", action="store_true",
+                    help="short run for rocprofv3: one A and one AT in each layout (z_order 0, "
+                         "then 2), twice; prints no JSON")
     ap.add_argument("--no-z2", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -299,6 +302,19 @@ def main():
     def step():
         _CU.forward_z0(x, y, ang)
         _CU.backward_z0(yb, xb, ang)
+
+    if args.profile_z2:
+        x2 = x.permute(2, 0, 1).contiguous()
+        yb2 = yb.permute(2, 0, 1).contiguous()
+        y2 = torch.zeros((N, A, PX), device=dev)
+        xb2 = torch.zeros((N, N, N), device=dev)
+        for _ in range(2):
+            step()
+            _CU.forward_z2(x2, y2, ang)
+            _CU.backward_z2(yb2, xb2, ang)
+        torch.cuda.synchronize()
+        print("profile run done: A z0, AT z0, A z2, AT z2, twice", flush=True)
+        return
 
     if args.profile:
         step()
