@@ -24,14 +24,18 @@ DEV = "cuda"
 # (SY, SX, H, n_angles, PX): SX a multiple of 4 with whole and partial tile
 # rows, SX odd (every row on the single-float path), H just above 128 and
 # not a multiple of 4 or 256
-SHAPES = [(48, 64, 260, 37, 70), (41, 52, 300, 29, 60), (33, 37, 131, 23, 45), (20, 24, 513, 11, 30)]
+SHAPES = [(48, 64, 260, 37, 70), (41, 52, 300, 29, 60), (33, 37, 131, 23, 45), (20, 24, 513, 11, 30),
+          # degenerate extents: one voxel, one row or column, a detector of one pixel
+          (1, 1, 129, 7, 3), (1, 8, 200, 5, 9), (9, 1, 257, 6, 1), (2, 4, 300, 40, 2)]
 
 
 def _z0_reference(y2, ang, SY, SX, x2_init):
     """AT in z_order 0 on the transposed sinogram, added to the initial
     volume in z_order 0 layout, returned in z_order 2 layout."""
-    y0 = y2.permute(1, 2, 0).contiguous()               # (A, PX, H)
-    x0 = x2_init.permute(1, 2, 0).contiguous()          # (SY, SX, H)
+    # (copies: for SY = SX = 1 the permuted volume is already contiguous, and
+    # .contiguous() would hand back a view of x2_init)
+    y0 = y2.permute(1, 2, 0).clone(memory_format=torch.contiguous_format)       # (A, PX, H)
+    x0 = x2_init.permute(1, 2, 0).clone(memory_format=torch.contiguous_format)  # (SY, SX, H)
     x0 = ops.backproject(y0, ang, SY, SX, 0, out=x0)
     return x0.permute(2, 0, 1).contiguous()
 
@@ -51,7 +55,9 @@ def test_direct_z2_equals_transposed_z0(shape):
     # against the FP64 oracle on the sums alone
     sums = (got - x_init).cpu().numpy().astype(np.float64)
     want = vpo.backward_f64(y2.cpu().numpy().astype(np.float64), ang.cpu().numpy(), SY, SX, 2)
-    assert np.linalg.norm(sums - want) / np.linalg.norm(want) <= 1e-6
+    # the sums are taken back out of a volume of N(0,1) values: allow that rounding
+    scale = max(np.linalg.norm(want), 1e-30)
+    assert np.linalg.norm(sums - want) / scale <= 1e-6 + 2e-7 * np.linalg.norm(x_init.cpu().numpy()) / scale
 
 
 def test_direct_z2_in_slabs_of_a_caller_workspace():
