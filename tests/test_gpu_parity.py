@@ -125,6 +125,10 @@ CASES = [
     (16, 16, 3, 1, 1, "linspace"),          # single pixel, single angle
     (40, 36, 131, 30, 44, "random"),        # 4 slices per lane, H % 4 != 0 (element DMA), ragged last block
     (24, 30, 200, 25, 30, "linspace"),      # 4 slices per lane, 16-byte DMA, idle lanes past H
+    (1, 1, 257, 7, 3, "random"),            # 4 slices per lane: single voxel, two slice blocks
+    (1, 9, 300, 5, 11, "linspace"),         # 4 slices per lane: one row (z_order 2 AT writes single floats)
+    (13, 1, 260, 9, 1, "random"),           # 4 slices per lane: one column, one-pixel detector
+    (3, 64, 256, 16, 70, "wide"),           # 4 slices per lane: z_order 2 AT on whole 16-byte tile rows
 ]
 
 
