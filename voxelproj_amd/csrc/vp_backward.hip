@@ -984,7 +984,6 @@ __device__ __forceinline__ OpArgs item_args(const BatchItem& it)
     g.PX = it.PX;
     g.accumulate = it.accumulate;
     g.out_z2 = 0;
-    g.tables_low = 0;
     const int bits = hi_bits(it.SY, it.SX, it.PX);
     g.du = nullptr;
     g.scale = ldexp(1.0, bits);
@@ -1291,7 +1290,6 @@ OpArgs backward_args(const float* y, float* x, const float* angles, const Geom& 
     a.PX = (int)g.PX;
     a.accumulate = accumulate;
     a.out_z2 = out_z2;
-    a.tables_low = 0;
     a.scale = ldexp(1.0, g.bits);
     a.inv_scale = ldexp(1.0, -g.bits);
     a.du = g.du;

@@ -74,24 +74,6 @@ struct FwdRoles {
     static constexpr int THREADS = 64 * (NW + NS);
 };
 
-// Staging priority of the slot tables.  The staging waves run above the
-// sweep waves; building the tables at the base priority instead lets the
-// sweep go first, which pays unless the staging stream is the critical path:
-// that is when a block's line segments are long (NW consecutive angles of A
-// over [0, pi) move a voxel at distance R = half the larger extent by
-// NW * (pi / A) * R pixels, much of the strip width P or more) and the grid
-// has several blocks per CU.  Measured (A, one MI355X): 512^3 x 192 (spread
-// 50 pixels, 768 blocks) 4.49 -> 4.61 ms with base-priority tables;
-// 384^2 x 256 x 360 2.86 -> 2.69, 256^2 x 64 x 1024 1.97 -> 1.75,
-// 512^2 x 128 x 720 5.96 -> 5.80, 1024^2 x 512 x 1800 138.1 -> 137.1,
-// 256^3 x 192 (192 blocks) 0.80 -> 0.77, 128^3 x 192 0.254 -> 0.237 ms.  Results
-// are the same either way.
-__host__ __device__ inline bool tables_at_base_priority(int P, int NW, int A, int64_t SY, int64_t SX)
-{
-    const float spread = (float)NW * 3.14159265f / (float)(A > 0 ? A : 1) * 0.5f * (float)(SY > SX ? SY : SX);
-    return spread < (float)P;
-}
-
 template <int P, int NW, int NS, int HL, int UMAX>
 __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical)
 {
@@ -594,31 +576,31 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
         if (t.row >= 0) r.d = plan(t.row, t.j0, t.j1, pass);
         return r;
     };
-    // Publish piece pt.t (or the end marker) in buffer b: its slot tables,
-    // then its DMA; then, while the DMA lands, find and plan the next piece
-    // (work on the wave's own registers), wait for the DMA and the table
-    // writes, and barrier.  Returns the next piece.  (The DMA split in three
-    // parts around the table rounds, with the next piece found after the
-    // barrier, was measured 6 % slower at 512^3 x 192.)
+    // Publish piece pt.t (or the end marker) in buffer b: issue its DMA,
+    // then, at the base issue priority so that the sweep waves' FMAs go
+    // first, build its slot tables and find and plan the next piece (work on
+    // the wave's own registers) while the DMA lands; wait for the DMA and the
+    // table writes, barrier, and return the next piece.  Measured at
+    // 512^3 x 192 (A): DMA split in three parts around the table rounds and
+    // the next piece found after the barrier 4.73 ms; tables, DMA, then the
+    // next piece, all at the staging priority, 4.46 ms; this order 4.28 ms
+    // (and 3-8 % faster at 128^3 - 512^2 x 128 x 720).  Results are the same
+    // in every order.
     auto publish = [&](const Planned& pt, float4* xdst, const int b) __attribute__((always_inline)) {
         const Task& t = pt.t;
         if (t.row >= 0) {
             const Dma& d = pt.d;
             const int nmine = d.nins > s ? (d.nins - s + NS - 1) / NS : 0;  // this wave's instructions
-            if (g.tables_low) {
-                __builtin_amdgcn_s_setprio(0);
-                tables(t, b);
-                __builtin_amdgcn_s_setprio(1);
-            } else {
-                tables(t, b);
-            }
             for (int k = 0; k < nmine; ++k) issue(d, xdst, s + k * NS);
+            __builtin_amdgcn_s_setprio(0);  // the tables and the next piece: after the sweep's issue
+            tables(t, b);
         }
         if (s == 0 && lane == 0) s_rec[b] = make_int4(t.row, t.j0, t.j1, pass);
         // (next_piece may enter the next pass: after the record of t)
         const Planned nx = with_plan(t.row >= 0 ? next_piece(t) : t);
         __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA and table writes have landed
         __syncthreads();
+        __builtin_amdgcn_s_setprio(1);
         return nx;
     };
     Planned pt = with_plan(first_from_pass(0));
@@ -639,7 +621,6 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
 // range of logical blocks instead was measured 25 % slower.)
 __device__ __forceinline__ int xcd_logical(int bid, int /*nb*/) { return bid; }
 
-template <int P, int NW>
 __device__ __forceinline__ OpArgs item_args(const BatchItem& it)
 {
     OpArgs g;
@@ -653,7 +634,6 @@ __device__ __forceinline__ OpArgs item_args(const BatchItem& it)
     g.PX = it.PX;
     g.accumulate = it.accumulate;
     g.out_z2 = 0;
-    g.tables_low = tables_at_base_priority(P, NW, it.A, it.SY, it.SX);
     const int bits = hi_bits(it.SY, it.SX, it.PX);
     g.du = nullptr;
     g.scale = ldexp(1.0, bits);
@@ -685,7 +665,7 @@ __global__ __launch_bounds__((FwdRoles<NW, NS>::THREADS), WPE) void forward_batc
             if (w < base + nbl) break;
             base += nbl;
         }
-        forward_block<P, NW, NS, HL, UMAX>(item_args<P, NW>(items[i]), (int)(w - base));
+        forward_block<P, NW, NS, HL, UMAX>(item_args(items[i]), (int)(w - base));
         __syncthreads();
     }
 }
@@ -818,14 +798,12 @@ struct FwdSmall {
 };
 
 template <int HL, typename C = FwdCfg<HL>>
-hipError_t forward_single(const OpArgs& g0, hipStream_t s)
+hipError_t forward_single(const OpArgs& g, hipStream_t s)
 {
     int nbp, nba, nbh;
-    forward_grid<C::P, C::NW, HL>(g0.A, g0.H, g0.PX, nbp, nba, nbh);
+    forward_grid<C::P, C::NW, HL>(g.A, g.H, g.PX, nbp, nba, nbh);
     const int64_t nb = (int64_t)nbp * nba * nbh;
     if (nb > 0x7fffffff) return hipErrorInvalidValue;
-    OpArgs g = g0;
-    g.tables_low = tables_at_base_priority(C::P, C::NW, g.A, g.SY, g.SX) || nb < 2 * (int64_t)device_cus();
     hipLaunchKernelGGL((forward_z0_kernel<C::P, C::NW, C::NS, HL, C::UMAX, C::WPE>), dim3((unsigned)nb),
                        dim3(FwdRoles<C::NW, C::NS>::THREADS), 0, s, g);
     return hipGetLastError();
@@ -865,7 +843,6 @@ hipError_t launch_forward_z0(const float* x, float* y, const float* angles, cons
     a.PX = (int)g.PX;
     a.accumulate = accumulate;
     a.out_z2 = 0;
-    a.tables_low = 0;  // set per configuration in forward_single
     a.scale = ldexp(1.0, g.bits);
     a.inv_scale = ldexp(1.0, -g.bits);
     a.du = g.du;

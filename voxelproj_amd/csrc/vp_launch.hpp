@@ -23,7 +23,6 @@ struct OpArgs {
     double scale, inv_scale;  // 2^B and 2^-B of the cos/sin hi split
     const float* du;          // per-angle detector translation (pixels), or null
     int out_z2;               // AT, 4 slices per lane: out is (H, SY, SX) instead of (SY, SX, H)
-    int tables_low;           // A: staging waves build slot tables at the base issue priority
 };
 
 // One instance of a batched launch (layout shared with the C ABI's vp_batch_item).
