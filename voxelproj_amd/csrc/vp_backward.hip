@@ -486,7 +486,13 @@ __device__ __forceinline__ void backproject4_block(const OpArgs& g, const int lo
             tables(a0 + APL, tab_of(a0 + APL));
             __syncthreads();
         }
-        // footprint of (voxel lv, angle a0 + la); the group lies in one table range
+        // footprint of (voxel lv, angle a0 + la); the group lies in one table
+        // range.  This per-group part (a dependent chain ending in the DMA
+        // rows of the group's chunks) runs at a raised issue priority, the
+        // tap loop at the base one, so a wave entering its group is not held
+        // up by the other waves' FMAs (256^3 x 192: 0.45 -> 0.40 ms, 512^3:
+        // 3.04 -> 3.01 ms; same results).
+        __builtin_amdgcn_s_setprio(1);
         const int al = a0 + la;
         const bool aval = al < A;
         const int ae = ent_of(min(al, A - 1));
@@ -515,6 +521,7 @@ __device__ __forceinline__ void backproject4_block(const OpArgs& g, const int lo
             if (lane < NSUB * RPW && row < AC * WIN) rows_l = dma_row(gi * NSUB + sub + 1, row);
         }
 
+        __builtin_amdgcn_s_setprio(0);
         // The group's first item of voxel v (sub 0, i = v; chunk gi * NSUB
         // always holds an angle) sets part[v] to its first product instead of
         // adding it to zeros: the sums differ at most in the sign of a zero,
@@ -843,7 +850,10 @@ __device__ __forceinline__ void backproject_rows_block(const OpArgs& g, const in
             tables(a0 + APL, tab_of(a0 + APL));
             __syncthreads();
         }
-        // footprint of (voxel lv, angle a0 + la); the group lies in one table range
+        // footprint of (voxel lv, angle a0 + la); the group lies in one table
+        // range (at a raised issue priority, as in the 4-slice kernel:
+        // 512^2 x 64 x 360: 1.01 -> 0.90 ms, 128^3: 0.116 -> 0.114 ms)
+        __builtin_amdgcn_s_setprio(1);
         uint64_t has3;  // bit L: entry L has a third tap
         {
             const int al = a0 + la;
@@ -862,6 +872,7 @@ __device__ __forceinline__ void backproject_rows_block(const OpArgs& g, const in
             has3 = __ballot(w3 != 0.0f);
         }
 
+        __builtin_amdgcn_s_setprio(0);
         float part[VPL][4];  // set by the group's first item of each voxel, as in the 4-slice kernel
 
 #pragma unroll
