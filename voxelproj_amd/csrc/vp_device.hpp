@@ -190,4 +190,24 @@ __device__ __forceinline__ void dma16_to_lds(__amdgpu_buffer_rsrc_t r, uint32_t 
     __builtin_amdgcn_raw_ptr_buffer_load_lds(r, lds, 16, voff, soff, 0, 0);
 }
 
+// An LDS array that a wave reads uniformly in an unrolled loop (one element
+// per step), addressed from a VGPR that holds its base.  LDS reads take a
+// VGPR address; left to itself the compiler keeps a uniform base in an SGPR
+// and copies it into a VGPR before every read (one VALU instruction per
+// step).  The copy made here is an asm result, which the compiler keeps
+// rather than recomputes, and each read folds its element offset into the
+// instruction.  `p` must point into LDS and be wave-uniform.  (Components of
+// these vector types are read as values: __builtin_bit_cast of a component
+// expression reinterprets the vector's first element instead.)
+typedef float pair_f32 __attribute__((ext_vector_type(2)));
+typedef float quad_f32 __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(3))) const T* lds_vgpr_base(const void* p)
+{
+    const uint32_t s = (uint32_t)(uintptr_t)(__attribute__((address_space(3))) const void*)p;
+    uint32_t v;
+    asm("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
+    return (__attribute__((address_space(3))) const T*)(uintptr_t)v;
+}
+
 }  // namespace vp

@@ -312,37 +312,72 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                 // the register budget).  The second voxel is the next staged
                 // slot, so both reads share one address and the second takes a
                 // constant instruction offset.
-                auto slot_src = [&](int q) { return sx + __builtin_bit_cast(int, rdlane(t0.w, q)); };
-                // The rank-0 weights (wl, wr) of pixel q+PF are read PF pixels
-                // ahead, as a wave-uniform LDS read: VGPR operands, for the
-                // faster all-VGPR form of the FMAs and no v_readlane.
-                const float2* wtab = reinterpret_cast<const float2*>(&s_tab[b][wave][0][0]);
                 constexpr int PF = HL == 1 ? 2 : 1;  // slots read ahead
+                // Weights and row offsets: the rank-0 weights (wl, wr) of
+                // pixel q+PF are read PF pixels ahead, as a wave-uniform LDS
+                // read: VGPR operands, for the faster all-VGPR form of the FMAs
+                // and no v_readlane.  The table's base stays in a VGPR
+                // (512^3 x 192: 4.28 -> 4.23 ms; 128^3 0.235 -> 0.220 ms).
+                // At 1 slice per lane the whole record (wl, wr, w3, offset) is
+                // read that way, PF + 1 pixels ahead, and gives the address of
+                // the slot's rows without a v_readlane (256^2 x 32 x 360:
+                // 0.936 -> 0.869 ms); at 4 slices per lane that form was 2.5 %
+                // slower (512^3 x 192), and the offset comes by v_readlane from
+                // the lane's own table entry.
+                constexpr bool kRecords = HL == 1;
+                constexpr int NR = kRecords ? PF + 2 : PF;  // table entries in flight
+                const auto wtab = lds_vgpr_base<pair_f32>(&s_tab[b][wave][0][0]);
+                const auto rtab = lds_vgpr_base<quad_f32>(&s_tab[b][wave][0][0]);
                 using V = decltype(ld(sx));
-                struct Pend {
-                    V xa, xb;
-                    float2 wv;
+                V xar[PF], xbr[PF];
+                pair_f32 wv[NR];                 // (wl, wr) of the entries in flight
+                float offv[kRecords ? NR : 1];   // their row offsets (records only)
+                float w3v[kRecords ? NR : 1];    // and third weights
+                auto fetch_entry = [&](const int q) {
+                    if constexpr (kRecords) {
+                        const quad_f32 r = rtab[q];
+                        wv[q % NR] = r.xy;
+                        w3v[q % NR] = r.z;
+                        offv[q % NR] = r.w;
+                    } else {
+                        wv[q % NR] = wtab[2 * q];  // float4 entries: (wl, wr) at float2 index 2q
+                    }
                 };
-                Pend ring[PF];
+                auto slot_src = [&](const int q) {
+                    if constexpr (kRecords) return sx + __float_as_int(offv[q % NR]);
+                    else return sx + __float_as_int(rdlane(t0.w, q));
+                };
+                auto fetch_rows = [&](const int q) {
+                    const float* s0 = slot_src(q);
+                    xar[q % PF] = ld(s0);
+                    xbr[q % PF] = ld(s0 + VOX);
+                };
+                // (a record before the rows it addresses; the rows before the
+                // weights otherwise)
+                if constexpr (kRecords) {
+#pragma unroll
+                    for (int dd = 0; dd <= PF; ++dd)
+                        if (dd < SLOTS) fetch_entry(dd);
+                }
 #pragma unroll
                 for (int dd = 0; dd < PF; ++dd) {
                     if (dd < SLOTS) {
-                        const float* s0 = slot_src(dd);
-                        ring[dd].xa = ld(s0);
-                        ring[dd].xb = ld(s0 + VOX);
-                        ring[dd].wv = wtab[2 * dd];
+                        fetch_rows(dd);
+                        if constexpr (!kRecords) fetch_entry(dd);
                     }
                 }
 #pragma unroll
                 for (int q = 0; q < SLOTS; ++q) {
-                    const auto xa = ring[q % PF].xa;
-                    const auto xb = ring[q % PF].xb;
-                    const float wl = ring[q % PF].wv.x, wr = ring[q % PF].wv.y;
-                    if (q + PF < SLOTS) {
-                        const float* s1 = slot_src(q + PF);
-                        ring[q % PF].xa = ld(s1);
-                        ring[q % PF].xb = ld(s1 + VOX);
-                        ring[q % PF].wv = wtab[2 * (q + PF)];  // float4 entries: (wl, wr) at float2 index 2q
+                    const auto xa = xar[q % PF];
+                    const auto xb = xbr[q % PF];
+                    const float wl = wv[q % NR].x, wr = wv[q % NR].y;
+                    const float w3a = kRecords ? w3v[q % NR] : 0.f;
+                    if constexpr (kRecords) {
+                        if (q + PF + 1 < SLOTS) fetch_entry(q + PF + 1);
+                        if (q + PF < SLOTS) fetch_rows(q + PF);
+                    } else if (q + PF < SLOTS) {
+                        fetch_rows(q + PF);
+                        fetch_entry(q + PF);
                     }
 #pragma unroll
                     for (int j = 0; j < HL; ++j) {
@@ -353,7 +388,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                     // three-pixel rank-0 footprint nor a second voxel
                     if ((extra >> q) & 1u) {
                         if (q < P && ((has3a >> q) & 1u)) {
-                            const float w3 = rdlane(t0.z, q);
+                            const float w3 = kRecords ? w3a : rdlane(t0.z, q);
 #pragma unroll
                             for (int j = 0; j < HL; ++j) part[q][j] = fmaf(w3, xa.a[j], part[q][j]);
                         }
