@@ -170,7 +170,8 @@ VP_API int vp_vshift(int z_order, int transpose, const float *in, float *out, in
  * to size the launch) and `items_dev` (an identical copy in device memory,
  * read by the kernel).  `variant` selects the kernel configuration (1, 2 or 4
  * slices per lane for either operator; 32, 64 or 128 slices per block for
- * AT); see vp_default_variant.  `grid` is the number of persistent workgroups
+ * AT; 32 for A: 1 slice per lane and two angles per wave, 32 slices per
+ * block); see vp_default_variant.  `grid` is the number of persistent workgroups
  * (<= 0: one per work item).  Results do not depend on the grid or on the
  * other instances in the batch.  Because the launch reads only device
  * memory, it can be captured into a HIP graph and replayed.
@@ -221,7 +222,9 @@ VP_API int vp_transpose_batched(const vp_transpose_item *items_host,
  * launches with fewer workgroups than compute units.  Variants 1 and 2 give
  * identical results; 4, 32, 64 and 128 identical results among themselves;
  * the two families add the same terms in another order (agreement to
- * float32 rounding). */
+ * float32 rounding).  A's variant 32 (never a default; for the autotuner on
+ * batches of H <= 32) equals its variant 1 on volumes of at most 128 voxels
+ * per side and agrees with it to float32 rounding on larger ones. */
 VP_API int vp_default_variant(int op, int64_t H);
 
 /* Number of work items (workgroups) one instance contributes to a batched

@@ -183,3 +183,32 @@ def test_concurrent_groups_equal_sequential():
         torch.cuda.synchronize()
         assert all(torch.equal(u, v) for u, v in zip(ys, ref_y)), rep
         assert all(torch.equal(u, v) for u, v in zip(xb, ref_x)), rep
+
+
+def test_narrow_forward_instances_form_their_own_group():
+    """Forward instances with H <= 32 form their own group (variant 1, the
+    single-instance operator's, so the grouped result equals it bit for
+    bit); switching that group to variant 32 leaves the other groups alone
+    and agrees with the plain operator to float32 rounding."""
+    xs, _, angs = make_instances(9)
+    ys = [torch.empty((a.numel(), s[4], s[2]), device=DEV) for a, s in zip(angs, SHAPES)]
+    pa = batch.GroupedPlan("A", xs, ys, angs)
+    narrow = [p for p in pa.plans if all(sh[2] <= 32 for sh in p.shapes)]
+    assert len(narrow) == 1 and narrow[0].variant == 1
+    assert sorted(sh[2] for sh in narrow[0].shapes) == sorted(s[2] for s in SHAPES if s[2] <= 32)
+    assert all(all(sh[2] > 32 for sh in p.shapes) for p in pa.plans if p is not narrow[0])
+    pa.launch()
+    torch.cuda.synchronize()
+    for x, a, y, s in zip(xs, angs, ys, SHAPES):
+        assert torch.equal(y, ops.project(x, a, s[4]))
+    ref = [t.clone() for t in ys]
+    narrow[0].variant = 32
+    for t in ys:
+        t.fill_(float("nan"))
+    pa.launch()
+    torch.cuda.synchronize()
+    for y, r, s in zip(ys, ref, SHAPES):
+        if s[2] > 32 or max(s[0], s[1]) <= 128:
+            assert torch.equal(y, r), s
+        else:
+            assert rel(y, r) <= 1e-6, s

@@ -81,6 +81,39 @@ def test_small_configurations_match_oracle(shape):
         assert np.abs(got - ref).max() / np.abs(ref).max() <= 1e-5
 
 
+# Forward variant 32 (1 slice per lane, two angles per sweep wave, 32
+# slices per block): H <= 32, H > 32 over several slice blocks, ragged H,
+# one voxel, and volumes over 128 voxels per side (lines split into pieces).
+TWO_ANGLE_SHAPES = [(24, 20, 9, 30, 32), (96, 96, 32, 120, 96), (128, 128, 32, 192, 128),
+                    (100, 60, 70, 50, 110), (1, 1, 16, 7, 3), (33, 41, 30, 37, 45),
+                    (72, 200, 31, 77, 150), (256, 256, 24, 360, 256)]
+
+
+@pytest.mark.parametrize("shape", TWO_ANGLE_SHAPES)
+def test_forward_two_angles_per_wave(shape):
+    """Variant 32 against variant 1: bit for bit where no line is split
+    (at most 128 voxels per side), to float32 rounding elsewhere; a few
+    slices against the FP64 oracle, with accumulation into a nonzero
+    sinogram."""
+    SY, SX, H, A, PX = shape
+    print(f"two angles per wave {shape}", flush=True)
+    x, y0, ang = _inputs(shape, 29)
+    ang = torch.tensor(np.random.default_rng(A).uniform(0, 2 * np.pi, A).astype(np.float32), device=DEV)
+    y1, y32 = y0.clone(), y0.clone()
+    batch.BatchPlan("A", [x], [y1], [ang], accumulate=True, variant=1).launch()
+    batch.BatchPlan("A", [x], [y32], [ang], accumulate=True, variant=32).launch()
+    if max(SY, SX) <= 128:
+        assert torch.equal(y32, y1)
+    else:
+        d = (y32 - y0).double()
+        r = (y1 - y0).double()
+        assert float((d - r).norm() / r.norm()) <= 1e-6
+    hs = slice(0, min(H, 4))
+    ref = vpo.forward_f64(x[:, :, hs].contiguous().cpu().numpy(), ang.cpu().numpy(), PX, 0)
+    got = (y32 - y0)[:, :, hs].cpu().numpy().astype(np.float64)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-6
+
+
 # x-dominant angles with cos > 0 and cos < 0 and y-dominant angles with
 # sin > 0 and sin < 0, interleaved, so every 8- or 12-angle block of the
 # forward kernel holds, in both passes, waves whose centre pixel runs

@@ -33,7 +33,7 @@ import torch
 
 from . import _lib, ops, recon
 
-FORWARD_VARIANTS = (1, 2, 4)
+FORWARD_VARIANTS = (1, 2, 4, 32)
 BACKWARD_VARIANTS = (1, 2, 4, 32, 64, 128)
 
 
@@ -256,6 +256,10 @@ class GroupedPlan:
     for -- the one the single-instance operator uses, so each result equals
     the plain operator's bit for bit -- while each group still runs as one
     persistent launch.  ``variant_of(H)`` overrides the grouping rule.
+    Forward instances with H <= 32 form a group of their own (variant 1 by
+    default, like the single-instance operator), so that ``autotune_groups``
+    can give them variant 32 (two angles per sweep wave: no idle lanes at
+    H <= 32) without changing the variant of the H = 33..64 instances.
     """
 
     def __init__(self, op, ins, outs, angles, accumulate=False, grid=None, variant_of=None,
@@ -271,12 +275,13 @@ class GroupedPlan:
             # H: the last extent of volumes and sinograms in z_order 0, the first in 2
             H = int(t.shape[2] if z_order == 0 else t.shape[0])
             v = int(variant_of(H)) if variant_of is not None else int(lib.vp_default_variant(code, H))
-            groups.setdefault(v, []).append(i)
+            narrow = variant_of is None and op == "A" and H <= 32
+            groups.setdefault((v, narrow), []).append(i)
         self.op = op
         self.device = ins[0].device
         self.plans = [BatchPlan(op, [ins[i] for i in idx], [outs[i] for i in idx],
                                 [angles[i] for i in idx], accumulate, v, grid, z_order)
-                      for v, idx in sorted(groups.items())]
+                      for (v, _), idx in sorted(groups.items())]
         self.outs = list(outs)
         self._streams = None
 
@@ -286,7 +291,11 @@ class GroupedPlan:
 
     @property
     def variants(self):
-        return {p.variant: p.n for p in self.plans}
+        """{variant: number of instances} over the groups."""
+        out = {}
+        for p in self.plans:
+            out[p.variant] = out.get(p.variant, 0) + p.n
+        return out
 
     def launch(self, concurrent=False):
         """One launch per group on the current stream; with concurrent=True

@@ -84,9 +84,10 @@ int validate_batch(int op, int variant, const vp_batch_item* items, int n)
         if (it.accumulate != 0 && it.accumulate != 1)
             return fail(VP_ERR_ARG, "batch item " + std::to_string(i) + ": accumulate must be 0 or 1");
         // The forward kernels with 1 or 2 slices per lane stage 4 or 2 voxels
-        // per DMA instruction through one 32-bit buffer offset.
-        if (op == 0 && (variant == 1 || variant == 2)) {
-            const int64_t span = (int64_t)(4 / variant - 1) * it.SX * it.H + it.H;
+        // per DMA instruction (variant 32: 8) through one 32-bit buffer offset.
+        if (op == 0 && (variant == 1 || variant == 2 || variant == 32)) {
+            const int64_t per = variant == 32 ? 8 : 4 / variant;  // voxels per instruction
+            const int64_t span = (per - 1) * it.SX * it.H + it.H;
             if (span >= ((int64_t)1 << 29))
                 return fail(VP_ERR_LIMIT, "batch item " + std::to_string(i) + ": SX * H too large for variant " +
                                               std::to_string(variant) + " (use variant 4)");

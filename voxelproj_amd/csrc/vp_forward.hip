@@ -58,23 +58,29 @@ struct Task {
 // tables, the DMA in flight) overlaps the sweep of the previous piece instead
 // of preceding it in every wave.  Angles whose dominant axis is x and those
 // whose dominant axis is y are handled in two passes over the volume.
-template <int P, int NW, int HL>
+//
+// With AW = 2 (1 slice per lane only) each sweep wave serves two angles, one
+// per half-wave of 32 lanes, so a block covers 2 NW angles x 32 slices: for
+// H <= 32, where a wave of 64 one-slice lanes would leave half its lanes
+// idle.  Every output still gets the same terms in the same order.
+template <int P, int NW, int HL, int AW = 1>
 __host__ __device__ inline void forward_grid(int A, int64_t H, int64_t PX, int& nbp, int& nba, int& nbh)
 {
+    constexpr int SB = 64 * HL / AW;  // slices per block
     nbp = (int)((PX + P - 1) / P);
-    nba = (A + NW - 1) / NW;
-    nbh = (int)((H + 64 * HL - 1) / (64 * HL));
+    nba = (A + NW * AW - 1) / (NW * AW);
+    nbh = (int)((H + SB - 1) / SB);
 }
 
-// A block of NW sweep waves and NS staging waves.
-template <int NW, int NS>
+// A block of NW sweep waves (AW angles each) and NS staging waves.
+template <int NW, int NS, int AW = 1>
 struct FwdRoles {
     static_assert(NW + NS <= 16, "at most 1024 threads per block");
-    static constexpr int AS = (NW + NS - 1) / NS;  // angles per staging wave
+    static constexpr int AS = (NW * AW + NS - 1) / NS;  // angles per staging wave
     static constexpr int THREADS = 64 * (NW + NS);
 };
 
-template <int P, int NW, int NS, int HL, int UMAX>
+template <int P, int NW, int NS, int HL, int UMAX, int AW = 1>
 __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical)
 {
     const float* __restrict__ x = g.in;
@@ -84,12 +90,14 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     const double scale = g.scale, inv_scale = g.inv_scale;
     const int accumulate = g.accumulate;
     int nbp, nba, nbh;
-    forward_grid<P, NW, HL>(A, H, PX, nbp, nba, nbh);
+    forward_grid<P, NW, HL, AW>(A, H, PX, nbp, nba, nbh);
     static_assert((P + 5) * 1.4143 + 4 <= kSeg, "segment may exceed one wave");
-    constexpr int AS = FwdRoles<NW, NS>::AS;
+    static_assert(AW == 1 || (AW == 2 && HL == 1), "two angles per wave only at 1 slice per lane");
+    constexpr int NA = NW * AW;                   // angles per block
+    constexpr int AS = FwdRoles<NW, NS, AW>::AS;
     constexpr int SLOTS = P + 2;                 // first pixels p0-2 .. p0+P-1
     static_assert(SLOTS <= 32, "slot masks are 32-bit");
-    constexpr int VOX = 64 * HL;                 // floats per staged voxel
+    constexpr int VOX = 64 * HL / AW;            // floats per staged voxel (slices per block)
     // floats per buffer: UMAX voxels plus one, so that the sweep's read of a
     // slot's second voxel (one voxel above the first) stays inside the array
     // even where the slot has none
@@ -103,11 +111,11 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     // Slot tables [buffer][angle][rank][slot]: slot q of rank r holds the r-th
     // voxel whose footprint starts at pixel p0 - 2 + q as (wl, wr, w3, LDS
     // offset of its staged slices); empty slots hold zeros.
-    __shared__ float4 s_tab[2][NW][2][SLOTS];
+    __shared__ float4 s_tab[2][NA][2][SLOTS];
     // piece k in s_rec[k % 2]: (line, first voxel, last voxel, pass); line -1 after the last
     __shared__ int4 s_rec[2];
-    __shared__ AngleK s_k[NW];
-    __shared__ int s_mode[NW];
+    __shared__ AngleK s_k[NA];
+    __shared__ int s_mode[NA];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -125,8 +133,8 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     const float yc = 0.5f * (float)(SY - 1);
     const bool vec4 = (H & 3) == 0;
 
-    if (tid < NW) {
-        const int at = ba * NW + tid;
+    if (tid < NA) {
+        const int at = ba * NA + tid;
         const int ac = min(at, A - 1);
         const AngleK kk = make_angle(angles[ac], g.du ? g.du[ac] : 0.0f, 0.5 * (double)(PX - 1), scale, inv_scale);
         s_k[tid] = kk;
@@ -137,14 +145,15 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
 
     // Staging DMA of a piece, issued by the staging waves: staging wave s
     // issues the instructions s, s + NS, ...; one instruction moves RPI
-    // voxels of 64*HL slices (16 B per lane when H is a multiple of 4; else
-    // HL dword instructions per voxel).  Idle lanes past H load a clamped,
-    // valid address; their results are never stored.  Staged slot v holds
-    // voxel j0 + v (the same order for every angle of the block, whatever its
-    // direction along the line).
+    // voxels of VOX slices (16 B per lane when H is a multiple of 4; else
+    // dword instructions: HL per voxel, or one per 64 / VOX voxels).  Idle
+    // lanes past H load a clamped, valid address; their results are never
+    // stored.  Staged slot v holds voxel j0 + v (the same order for every
+    // angle of the block, whatever its direction along the line).
     constexpr int RPI = 1024 / (VOX * 4);    // voxels per 16-byte-per-lane DMA
     constexpr int LPR = 64 / RPI;            // lanes per voxel
-    static_assert(UMAX % RPI == 0, "staging buffer must hold whole DMA groups");
+    constexpr int DPI = VOX >= 64 ? 1 : 64 / VOX;  // voxels per dword DMA
+    static_assert(UMAX % RPI == 0 && UMAX % DPI == 0, "staging buffer must hold whole DMA groups");
     struct Dma {
         const float* base;
         int64_t st_along;
@@ -160,7 +169,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
         const int64_t st_row = px ? (int64_t)SX * H : (int64_t)H;
         d.nv = j1 - j0 + 1;
         d.base = x + (int64_t)row * st_row + (int64_t)j0 * d.st_along;
-        d.nins = vec4 ? (d.nv + RPI - 1) / RPI : d.nv;
+        d.nins = vec4 ? (d.nv + RPI - 1) / RPI : (d.nv + DPI - 1) / DPI;
         // one descriptor for the piece when its extent fits 31 bits
         const int64_t extent = ((int64_t)(d.nv - 1) * d.st_along + H) * 4;
         d.fast = extent < (1ll << 31);
@@ -187,6 +196,17 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                                (uint32_t)(((int64_t)(RPI - 1) * d.st_along + H) * 4),
                                (uint32_t)(((int64_t)(v - v0) * d.st_along + hoff) * 4), lds);
             }
+        } else if constexpr (DPI > 1) {
+            // lane = (voxel v0 + lane / VOX, slice hb + lane % VOX); offsets
+            // below (DPI-1) * st_along + H floats, under 2^29 (DPI > 1 only
+            // at up to 32 slices)
+            const int v0 = ins * DPI;
+            const int v = min(v0 + lane / VOX, d.nv - 1);
+            const int hh = min(hb + lane % VOX, H - 1);
+            dma_to_lds<4>(d.base + (int64_t)v0 * d.st_along,
+                          (uint32_t)(((int64_t)(DPI - 1) * d.st_along + H) * 4),
+                          (uint32_t)(((int64_t)(v - v0) * d.st_along + hh) * 4),
+                          (__attribute__((address_space(3))) void*)&dst[v0 * (VOX / 4)]);
         } else {
 #pragma unroll
             for (int c = 0; c < HL; ++c) {
@@ -201,7 +221,11 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
 
     if (wave < NW) {
         // ---------------------------------------------------------------- sweep
-        const int a = ba * NW + wave;
+        // lane = (half-wave sub, slice group sl); half-wave sub serves angle
+        // wave * AW + sub of the block (AW = 1: the whole wave, sl = lane)
+        const int sub = AW > 1 ? lane / VOX : 0;
+        const int sl = AW > 1 ? lane % VOX : lane;
+        const int a = ba * NA + wave * AW + sub;
         // Register partial sums over at most kRowsPerFold lines of one pass;
         // the owning wave folds them into y (it is the only writer of its
         // outputs), which keeps the FP32 summation two-level without a second
@@ -214,7 +238,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
         bool folded = false;
         auto fold = [&]() {
             if (a < A) {
-                const int hh0 = hb + lane * HL;
+                const int hh0 = hb + sl * HL;
                 const bool full = vec4 && hh0 + 3 < H;
 #pragma unroll
                 for (int i = 0; i < P; ++i) {
@@ -258,11 +282,13 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
             // latencies overlap: the piece's record and its slot table
             // (unmasked reads; lanes past the slots drop theirs)
             const int4 rc = s_rec[b];
-            const int ql = lane < SLOTS ? lane : SLOTS - 1;
+            constexpr int TL = AW > 1 ? 32 : 64;  // lanes per angle of the table reads
+            const int tq = lane % TL;            // slot read by the lane
+            const int ql = tq < SLOTS ? tq : SLOTS - 1;
             const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-            float4 t0 = s_tab[b][wave][0][ql];
-            float4 t1 = s_tab[b][wave][1][ql];
-            if (lane >= SLOTS) {
+            float4 t0 = s_tab[b][wave * AW + lane / TL][0][ql];
+            float4 t1 = s_tab[b][wave * AW + lane / TL][1][ql];
+            if (tq >= SLOTS) {
                 t0 = zero;
                 t1 = zero;
             }
@@ -279,14 +305,21 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
             }
             if (swept) {
                 using Mask = uint32_t;
+                // per slot of either angle of the wave (AW = 2: the two halves'
+                // bits merged; a lane whose angle lacks the term applies a
+                // zero weight)
+                auto slot_bits = [](const uint64_t m) -> Mask {
+                    if constexpr (AW > 1) return (Mask)m | (Mask)(m >> 32);
+                    else return (Mask)m;
+                };
                 // bit q: slot q holds a second voxel (about 1 in 10 at 512^3 x
                 // 192); a voxel's first weight wl is always positive
-                const Mask has2 = (Mask)__ballot(t1.x != 0.f);
+                const Mask has2 = slot_bits(__ballot(t1.x != 0.f));
                 // bit q: the rank-0 / rank-1 voxel of slot q covers three pixels
-                const Mask has3a = (Mask)__ballot(t0.z != 0.f);
-                const Mask has3b = (Mask)__ballot(t1.z != 0.f);
+                const Mask has3a = slot_bits(__ballot(t0.z != 0.f));
+                const Mask has3b = slot_bits(__ballot(t1.z != 0.f));
                 const Mask extra = has3a | has2;  // has3b implies has2
-                const float* sx = reinterpret_cast<const float*>(xcur) + lane * HL;
+                const float* sx = reinterpret_cast<const float*>(xcur) + sl * HL;
                 auto ld = [&](const float* src) {
                     struct V { float a[HL]; } out;
                     if constexpr (HL == 4) {
@@ -325,9 +358,11 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                 // slower (512^3 x 192), and the offset comes by v_readlane from
                 // the lane's own table entry.
                 constexpr bool kRecords = HL == 1;
+                static_assert(kRecords || AW == 1, "per-lane records for two angles per wave");
                 constexpr int NR = kRecords ? PF + 2 : PF;  // table entries in flight
                 const auto wtab = lds_vgpr_base<pair_f32>(&s_tab[b][wave][0][0]);
-                const auto rtab = lds_vgpr_base<quad_f32>(&s_tab[b][wave][0][0]);
+                // the lane's angle's table (AW = 2: one per half-wave)
+                const auto rtab = lds_vgpr_base<quad_f32>(&s_tab[b][wave * AW][0][0]) + sub * 2 * SLOTS;
                 using V = decltype(ld(sx));
                 V xar[PF], xbr[PF];
                 pair_f32 wv[NR];                 // (wl, wr) of the entries in flight
@@ -393,15 +428,19 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                             for (int j = 0; j < HL; ++j) part[q][j] = fmaf(w3, xa.a[j], part[q][j]);
                         }
                         if ((has2 >> q) & 1u) {
-                            const float wl2 = rdlane(t1.x, q);
-                            const float wr2 = rdlane(t1.y, q);
+                            // rank 1 of slot q: lane q's table entry, or with
+                            // two angles per wave the lane's own angle's entry
+                            quad_f32 r1 = {0.f, 0.f, 0.f, 0.f};
+                            if constexpr (AW > 1) r1 = rtab[SLOTS + q];
+                            const float wl2 = AW > 1 ? (float)r1.x : rdlane(t1.x, q);
+                            const float wr2 = AW > 1 ? (float)r1.y : rdlane(t1.y, q);
 #pragma unroll
                             for (int j = 0; j < HL; ++j) {
                                 if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wl2, xb.a[j], part[q - 2][j]);
                                 if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(wr2, xb.a[j], part[q - 1][j]);
                             }
                             if (q < P && ((has3b >> q) & 1u)) {
-                                const float w3 = rdlane(t1.z, q);
+                                const float w3 = AW > 1 ? (float)r1.z : rdlane(t1.z, q);
 #pragma unroll
                                 for (int j = 0; j < HL; ++j) part[q][j] = fmaf(w3, xb.a[j], part[q][j]);
                             }
@@ -424,9 +463,9 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     // that the sweep waves' FMAs would otherwise delay, holding up the barrier.
     __builtin_amdgcn_s_setprio(1);
     const int s = wave - NW;  // staging wave s builds the tables of angles s*AS .. s*AS+AS-1
-    // Lane w < NW describes angle w of the block (segment union of a line).
-    const AngleK kl = s_k[lane < NW ? lane : 0];
-    const int model = (lane < NW) ? s_mode[lane] : -1;
+    // Lane w < NA describes angle w of the block (segment union of a line).
+    const AngleK kl = s_k[lane < NA ? lane : 0];
+    const int model = (lane < NA) ? s_mode[lane] : -1;
     // Slot-table lanes: lane = 32 * half + q builds slot q of angle 2 * pr + half
     // of this wave's AS angles in table round pr, so one instruction stream
     // serves two angles.
@@ -435,14 +474,14 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     const int half = lane >> 5;
     const int q = lane & 31;
     AngleK kv[ROUNDS];
-    int wv[ROUNDS], modev[ROUNDS];  // block angle of the lane per round (NW: none) and its pass
+    int wv[ROUNDS], modev[ROUNDS];  // block angle of the lane per round (NA: none) and its pass
 #pragma unroll
     for (int pr = 0; pr < ROUNDS; ++pr) {
         const int i = 2 * pr + half;
         const int w = s * AS + i;
-        const bool ok = i < AS && w < NW && q < SLOTS;
+        const bool ok = i < AS && w < NA && q < SLOTS;
         kv[pr] = s_k[ok ? w : 0];
-        wv[pr] = ok ? w : NW;
+        wv[pr] = ok ? w : NA;
         modev[pr] = ok ? s_mode[w] : -1;
     }
 
@@ -451,7 +490,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     bool px_ = true;
     int nrows = SY, nalong = SX;
     float ca = xc, cr = yc;
-    float kp_l = 0.f, ik_l = 0.f, ot_l = 0.f, pc_l = 0.f;  // lane w < NW: angle w
+    float kp_l = 0.f, ik_l = 0.f, ot_l = 0.f, pc_l = 0.f;  // lane w < NA: angle w
     float kapv[ROUNDS], ikapv[ROUNDS], othv[ROUNDS], pcv[ROUNDS];  // the lane's angle per round
     bool minev[ROUNDS];
     auto enter_pass = [&](int p) {
@@ -500,7 +539,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
         int lo = 0x7fffffff, hi = -1;
         // not unrolled: keeps the per-angle reads inside the loop
 #pragma unroll 1
-        for (int w = 0; w < NW; ++w) {
+        for (int w = 0; w < NA; ++w) {
             if (__builtin_amdgcn_readlane(model, w) == pass) {
                 int l, h;
                 segment(R, rdlane(kp_l, w), rdlane(ik_l, w), rdlane(ot_l, w), rdlane(pc_l, w), l, h);
@@ -590,7 +629,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                 c[e] = anchor_left(voxel_taps(px_ ? jc - xc : R, px_ ? R : jc - yc, k));
                 in[e] = minev[pr] && j >= t.j0 && j <= t.j1 && c[e].l == lq;
             }
-            if (wv[pr] < NW) {
+            if (wv[pr] < NA) {
                 const bool any = in[0] || in[1];
                 const Taps2 r0 = in[0] ? c[0] : c[1];
                 const int off = (j0 + (in[0] ? 0 : 1) - t.j0) * VOX;
@@ -676,17 +715,17 @@ __device__ __forceinline__ OpArgs item_args(const BatchItem& it)
     return g;
 }
 
-template <int P, int NW, int NS, int HL, int UMAX, int WPE>
+template <int P, int NW, int NS, int HL, int UMAX, int WPE, int AW>
 __global__ __launch_bounds__((FwdRoles<NW, NS>::THREADS), WPE) void forward_z0_kernel(OpArgs g)
 {
-    forward_block<P, NW, NS, HL, UMAX>(g, xcd_logical(blockIdx.x, gridDim.x));
+    forward_block<P, NW, NS, HL, UMAX, AW>(g, xcd_logical(blockIdx.x, gridDim.x));
 }
 
 // Persistent launch over a list of instances: block b handles work items
 // b, b + gridDim.x, ... of the concatenated (instance, block) list, in a
 // fixed order, so results do not depend on the grid or on the other
 // instances of the batch.
-template <int P, int NW, int NS, int HL, int UMAX, int WPE>
+template <int P, int NW, int NS, int HL, int UMAX, int WPE, int AW>
 __global__ __launch_bounds__((FwdRoles<NW, NS>::THREADS), WPE) void forward_batched_kernel(const BatchItem* __restrict__ items,
                                                                   int n, int64_t total)
 {
@@ -695,12 +734,12 @@ __global__ __launch_bounds__((FwdRoles<NW, NS>::THREADS), WPE) void forward_batc
         int64_t base = 0;
         for (; i < n - 1; ++i) {
             int nbp, nba, nbh;
-            forward_grid<P, NW, HL>(items[i].A, items[i].H, items[i].PX, nbp, nba, nbh);
+            forward_grid<P, NW, HL, AW>(items[i].A, items[i].H, items[i].PX, nbp, nba, nbh);
             const int64_t nbl = (int64_t)nbp * nba * nbh;
             if (w < base + nbl) break;
             base += nbl;
         }
-        forward_block<P, NW, NS, HL, UMAX>(item_args(items[i]), (int)(w - base));
+        forward_block<P, NW, NS, HL, UMAX, AW>(item_args(items[i]), (int)(w - base));
         __syncthreads();
     }
 }
@@ -804,20 +843,33 @@ __global__ __launch_bounds__(256) void transpose_batched_kernel(const TransposeI
     }
 }
 
-template <int HL>
+// Configuration of variant V: HL slices per lane, AW angles per sweep wave,
+// P pixels and NW sweep waves per block, NS staging waves, pieces of up to
+// UMAX voxels; WPE is the occupancy (waves per SIMD) the register budget
+// targets.
+template <int V>
 struct FwdCfg;
-// P pixels and NW angles per block, NS staging waves, pieces of up to UMAX
-// voxels; WPE is the occupancy (waves per SIMD) the register budget targets.
 // 1 and 2 slices per lane: 12 angles per block with 28-pixel strips.  Both
 // variants share P, NW, NS and UMAX, so they add every output's terms in the
 // same order (bitwise-equal results).
-template <> struct FwdCfg<1> { static constexpr int P = 28, NW = 12, NS = 4, UMAX = 128, WPE = 4; };
-template <> struct FwdCfg<2> { static constexpr int P = 28, NW = 12, NS = 4, UMAX = 128, WPE = 4; };
+template <> struct FwdCfg<1> { static constexpr int HL = 1, AW = 1, P = 28, NW = 12, NS = 4, UMAX = 128, WPE = 4; };
+template <> struct FwdCfg<2> { static constexpr int HL = 2, AW = 1, P = 28, NW = 12, NS = 4, UMAX = 128, WPE = 4; };
 // 4 slices per lane: 12 angles with 22-pixel strips (88 accumulators within
 // the 128 VGPRs of 16 waves per CU; the spills are fold addresses, outside
 // the sweep).  Pieces of up to 69 voxels: 140 KB of staging, 18 KB of slot
 // tables.
-template <> struct FwdCfg<4> { static constexpr int P = 22, NW = 12, NS = 4, UMAX = 69, WPE = 4; };
+template <> struct FwdCfg<4> { static constexpr int HL = 4, AW = 1, P = 22, NW = 12, NS = 4, UMAX = 69, WPE = 4; };
+// 32 slices per block (variant 32, for H <= 32): 1 slice per lane and two
+// angles per sweep wave, 24 angles per block, otherwise as variant 1.  Where
+// no line's staged union exceeds UMAX (volumes of at most 128 voxels per
+// side) the pieces are whole lines and the results equal variant 1's bit for
+// bit; on larger volumes the 24-angle union splits some lines elsewhere, and
+// an output whose voxels straddle a split in reverse order adds two of its
+// terms in the other order (float32 rounding).  It doubles the useful lanes
+// at H <= 32 but halves the grid and doubles each block's slot-table work,
+// so it pays on large grids only: batched launches offer it to the
+// autotuner; single launches keep variant 1.
+template <> struct FwdCfg<32> { static constexpr int HL = 1, AW = 2, P = 28, NW = 12, NS = 4, UMAX = 128, WPE = 4; };
 
 // Narrower strips for 1 slice per lane, used by single launches of volumes
 // of at most UMAX = 128 voxels per side when the default grid would have fewer
@@ -829,35 +881,34 @@ template <> struct FwdCfg<4> { static constexpr int P = 22, NW = 12, NS = 4, UMA
 // bit for bit.  (For larger volumes the strip width changes where a line's staged
 // union is split and which pieces pair up around a fold, hence the last bits.)
 struct FwdSmall {
-    static constexpr int P = 16, NW = 8, NS = 4, UMAX = 128, WPE = 6;
+    static constexpr int HL = 1, AW = 1, P = 16, NW = 8, NS = 4, UMAX = 128, WPE = 6;
 };
 
-template <int HL, typename C = FwdCfg<HL>>
+template <typename C>
 hipError_t forward_single(const OpArgs& g, hipStream_t s)
 {
     int nbp, nba, nbh;
-    forward_grid<C::P, C::NW, HL>(g.A, g.H, g.PX, nbp, nba, nbh);
+    forward_grid<C::P, C::NW, C::HL, C::AW>(g.A, g.H, g.PX, nbp, nba, nbh);
     const int64_t nb = (int64_t)nbp * nba * nbh;
     if (nb > 0x7fffffff) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((forward_z0_kernel<C::P, C::NW, C::NS, HL, C::UMAX, C::WPE>), dim3((unsigned)nb),
+    hipLaunchKernelGGL((forward_z0_kernel<C::P, C::NW, C::NS, C::HL, C::UMAX, C::WPE, C::AW>), dim3((unsigned)nb),
                        dim3(FwdRoles<C::NW, C::NS>::THREADS), 0, s, g);
     return hipGetLastError();
 }
 
-template <int HL>
+template <typename C>
 hipError_t forward_batched(const BatchItem* host, const BatchItem* dev, int n, int grid,
                            hipStream_t s)
 {
-    using C = FwdCfg<HL>;
     int64_t total = 0;
     for (int i = 0; i < n; ++i) {
         int nbp, nba, nbh;
-        forward_grid<C::P, C::NW, HL>(host[i].A, host[i].H, host[i].PX, nbp, nba, nbh);
+        forward_grid<C::P, C::NW, C::HL, C::AW>(host[i].A, host[i].H, host[i].PX, nbp, nba, nbh);
         total += (int64_t)nbp * nba * nbh;
     }
     if (total == 0) return hipSuccess;
     const int64_t gsz = grid > 0 ? std::min<int64_t>(grid, total) : total;
-    hipLaunchKernelGGL((forward_batched_kernel<C::P, C::NW, C::NS, HL, C::UMAX, C::WPE>), dim3((unsigned)gsz),
+    hipLaunchKernelGGL((forward_batched_kernel<C::P, C::NW, C::NS, C::HL, C::UMAX, C::WPE, C::AW>), dim3((unsigned)gsz),
                        dim3(FwdRoles<C::NW, C::NS>::THREADS), 0, s, dev, n, total);
     return hipGetLastError();
 }
@@ -884,10 +935,10 @@ hipError_t launch_forward_z0(const float* x, float* y, const float* angles, cons
     switch (single_variant(0, g.A, g.SY, g.SX, g.H, g.PX)) {
         case 1:
             if (std::max(g.SY, g.SX) <= FwdSmall::UMAX && forward_blocks(1, g.A, g.H, g.PX) < device_cus())
-                return forward_single<1, FwdSmall>(a, s);
-            return forward_single<1>(a, s);
-        case 2: return forward_single<2>(a, s);
-        default: return forward_single<4>(a, s);
+                return forward_single<FwdSmall>(a, s);
+            return forward_single<FwdCfg<1>>(a, s);
+        case 2: return forward_single<FwdCfg<2>>(a, s);
+        default: return forward_single<FwdCfg<4>>(a, s);
     }
 }
 
@@ -895,9 +946,10 @@ hipError_t launch_forward_batched(int variant, const BatchItem* items_host,
                                   const BatchItem* items_dev, int n, int grid, hipStream_t s)
 {
     switch (variant) {
-        case 1: return forward_batched<1>(items_host, items_dev, n, grid, s);
-        case 2: return forward_batched<2>(items_host, items_dev, n, grid, s);
-        case 4: return forward_batched<4>(items_host, items_dev, n, grid, s);
+        case 1: return forward_batched<FwdCfg<1>>(items_host, items_dev, n, grid, s);
+        case 2: return forward_batched<FwdCfg<2>>(items_host, items_dev, n, grid, s);
+        case 4: return forward_batched<FwdCfg<4>>(items_host, items_dev, n, grid, s);
+        case 32: return forward_batched<FwdCfg<32>>(items_host, items_dev, n, grid, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -909,6 +961,7 @@ int64_t forward_blocks(int variant, int A, int64_t H, int64_t PX)
         case 1: forward_grid<FwdCfg<1>::P, FwdCfg<1>::NW, 1>(A, H, PX, nbp, nba, nbh); break;
         case 2: forward_grid<FwdCfg<2>::P, FwdCfg<2>::NW, 2>(A, H, PX, nbp, nba, nbh); break;
         case 4: forward_grid<FwdCfg<4>::P, FwdCfg<4>::NW, 4>(A, H, PX, nbp, nba, nbh); break;
+        case 32: forward_grid<FwdCfg<32>::P, FwdCfg<32>::NW, 1, 2>(A, H, PX, nbp, nba, nbh); break;
         default: return -1;
     }
     return (int64_t)nbp * nba * nbh;
