@@ -5,7 +5,8 @@
 // footprint (the same per-voxel r and weights as the backprojector), gathered
 // into output-stationary accumulators:
 //   sweep wave = one angle x P consecutive detector pixels x 64*HL H slices
-//                (lane = HL consecutive slices);
+//                (lane = HL consecutive slices; variant 32: two angles x 32
+//                slices, one per half-wave);
 //   rows  = lines of the volume along the axis whose direction cosine is
 //           larger (|kappa| >= 1/sqrt 2), so consecutive voxels of a line
 //           are at least 1/sqrt 2 pixel apart on the detector and at most
