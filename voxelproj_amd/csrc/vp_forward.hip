@@ -115,6 +115,9 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     __shared__ float4 s_tab[2][NA][2][SLOTS];
     // piece k in s_rec[k % 2]: (line, first voxel, last voxel, pass); line -1 after the last
     __shared__ int4 s_rec[2];
+    // the piece after the one in buffer b (the staging waves' piece finder
+    // passes it on): (line, first voxel, last voxel, pass)
+    __shared__ int4 s_next[2];
     __shared__ AngleK s_k[NA];
     __shared__ int s_mode[NA];
 
@@ -463,14 +466,26 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     // Staging waves issue first: their work is short latency-bound chains
     // that the sweep waves' FMAs would otherwise delay, holding up the barrier.
     __builtin_amdgcn_s_setprio(1);
-    const int s = wave - NW;  // staging wave s builds the tables of angles s*AS .. s*AS+AS-1
+    const int s = wave - NW;  // staging wave s
+    // Piece finding: at 4 slices per lane, where the staging stream is the
+    // critical path of a piece and the other NS - 1 staging waves can build
+    // all NA slot tables in as many table rounds as NS waves would (NA = 12,
+    // NS = 4: two rounds either way), staging wave 0 alone finds and plans
+    // the next piece and passes it on through LDS, and builds no tables;
+    // otherwise every staging wave finds the pieces (the same sequence) and
+    // builds tables.  (At 1 and 2 slices per lane the sweep is the critical
+    // path, and the LDS hand-over only adds latency: 2-3 % slower there.)
+    constexpr int AS1 = NS > 1 ? (NA + NS - 2) / (NS - 1) : NA;
+    constexpr bool kFinder = HL == 4 && NS > 1 && (AS1 + 1) / 2 == (AS + 1) / 2;
+    constexpr int TAS = kFinder ? AS1 : AS;  // angles per table-building wave
+    const int tw = kFinder ? s - 1 : s;      // table wave tw builds angles tw*TAS .. tw*TAS+TAS-1
     // Lane w < NA describes angle w of the block (segment union of a line).
     const AngleK kl = s_k[lane < NA ? lane : 0];
     const int model = (lane < NA) ? s_mode[lane] : -1;
     // Slot-table lanes: lane = 32 * half + q builds slot q of angle 2 * pr + half
-    // of this wave's AS angles in table round pr, so one instruction stream
+    // of this wave's TAS angles in table round pr, so one instruction stream
     // serves two angles.
-    constexpr int ROUNDS = (AS + 1) / 2;
+    constexpr int ROUNDS = (TAS + 1) / 2;
     static_assert(SLOTS <= 32, "two angles' slots per wave");
     const int half = lane >> 5;
     const int q = lane & 31;
@@ -479,8 +494,8 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
 #pragma unroll
     for (int pr = 0; pr < ROUNDS; ++pr) {
         const int i = 2 * pr + half;
-        const int w = s * AS + i;
-        const bool ok = i < AS && w < NA && q < SLOTS;
+        const int w = tw * TAS + i;
+        const bool ok = tw >= 0 && i < TAS && w < NA && q < SLOTS;
         kv[pr] = s_k[ok ? w : 0];
         wv[pr] = ok ? w : NA;
         modev[pr] = ok ? s_mode[w] : -1;
@@ -668,14 +683,25 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
             const int nmine = d.nins > s ? (d.nins - s + NS - 1) / NS : 0;  // this wave's instructions
             for (int k = 0; k < nmine; ++k) issue(d, xdst, s + k * NS);
             __builtin_amdgcn_s_setprio(0);  // the tables and the next piece: after the sweep's issue
-            tables(t, b);
+            if (tw >= 0) tables(t, b);
         }
         if (s == 0 && lane == 0) s_rec[b] = make_int4(t.row, t.j0, t.j1, pass);
-        // (next_piece may enter the next pass: after the record of t)
-        const Planned nx = with_plan(t.row >= 0 ? next_piece(t) : t);
+        Planned nx;
+        if (!kFinder || s == 0) {
+            // (next_piece may enter the next pass: after the record of t)
+            nx = with_plan(t.row >= 0 ? next_piece(t) : t);
+            if (kFinder && lane == 0) s_next[b] = make_int4(nx.t.row, nx.t.j0, nx.t.j1, pass);
+        }
         __builtin_amdgcn_s_waitcnt(0);  // this wave's DMA and table writes have landed
         __syncthreads();
         __builtin_amdgcn_s_setprio(1);
+        if (kFinder && s != 0) {
+            // the piece wave 0 found, and its pass
+            const int4 nr = s_next[b];
+            const Task nt{uni(nr.x), uni(nr.y), uni(nr.z)};
+            if (nt.row >= 0 && uni(nr.w) != pass) enter_pass(uni(nr.w));
+            nx = with_plan(nt);
+        }
         return nx;
     };
     Planned pt = with_plan(first_from_pass(0));
