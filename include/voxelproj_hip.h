@@ -170,8 +170,9 @@ VP_API int vp_vshift(int z_order, int transpose, const float *in, float *out, in
  * to size the launch) and `items_dev` (an identical copy in device memory,
  * read by the kernel).  `variant` selects the kernel configuration (1, 2 or 4
  * slices per lane for either operator; 32, 64 or 128 slices per block for
- * AT; 32 for A: 1 slice per lane and two angles per wave, 32 slices per
- * block); see vp_default_variant.  `grid` is the number of persistent workgroups
+ * AT; for A 416, 418, 420: 4 slices per lane with 16-, 18- or 20-pixel
+ * strips, and 32: 1 slice per lane and two angles per wave, 32 slices per
+ * block); see vp_default_variant and vp_launch_variant.  `grid` is the number of persistent workgroups
  * (<= 0: one per work item).  Results do not depend on the grid or on the
  * other instances in the batch.  Because the launch reads only device
  * memory, it can be captured into a HIP graph and replayed.
@@ -218,14 +219,26 @@ VP_API int vp_transpose_batched(const vp_transpose_item *items_host,
  * lane (blocks of 64, 128, 256 slices); AT also has 32, 64, 128: blocks of
  * that many slices at 4 slices per lane, the wave's lanes in rows of 8, 16,
  * 32 that hold one voxel each (its default below 129 slices).  vp_forward /
- * vp_backward use the default too, except that A's 2 may drop to 1 for
- * launches with fewer workgroups than compute units.  Variants 1 and 2 give
- * identical results; 4, 32, 64 and 128 identical results among themselves;
- * the two families add the same terms in another order (agreement to
- * float32 rounding).  A's variant 32 (never a default; for the autotuner on
- * batches of H <= 32) equals its variant 1 on volumes of at most 128 voxels
- * per side and agrees with it to float32 rounding on larger ones. */
+ * vp_backward start from the default (see vp_launch_variant).  Variants 1
+ * and 2 give identical results; AT's 4, 32, 64 and 128 identical results
+ * among themselves; the two families add the same terms in another order
+ * (agreement to float32 rounding).  A also has 416, 418 and 420 (4 slices
+ * per lane with 16-, 18- and 20-pixel strips instead of 22) and 32 (1 slice
+ * per lane and two angles per wave, never a default: for the autotuner on
+ * batches of H <= 32; equal to variant 1 on volumes of at most 128 voxels per
+ * side).  A's 4-slice variants agree with each other, and 32 with 1, to
+ * float32 rounding. */
 VP_API int vp_default_variant(int op, int64_t H);
+
+/* The variant a single vp_forward / vp_backward call of this geometry uses,
+ * so that a batched launch grouped by it gives the same results bit for bit:
+ * the default of vp_default_variant, except that for A 2 slices per lane
+ * drops to 1 when its grid has fewer workgroups than the device has compute
+ * units, and 4 slices per lane takes the strip width (4, 416, 418, 420) whose
+ * grid fills the device best.  Uses the current device's compute-unit count
+ * (256 without a device).  -1 on invalid input.  No counterpart in the
+ * reference. */
+VP_API int vp_launch_variant(int op, int n_angles, int64_t SY, int64_t SX, int64_t H, int64_t PX);
 
 /* Number of work items (workgroups) one instance contributes to a batched
  * launch of `op` (0 = A, 1 = AT) under `variant`; < 0 on invalid input. */
@@ -235,7 +248,7 @@ VP_API int64_t vp_batch_blocks(int op, int variant, int n_angles, int64_t SY, in
 /* Description of the last failure on the calling thread ("" if none). */
 VP_API const char *vp_last_error(void);
 
-/* ABI version of this header (incremented on incompatible changes). */
+/* ABI version of this header (incremented whenever the exported interface changes). */
 VP_API int vp_abi_version(void);
 
 #ifdef __cplusplus

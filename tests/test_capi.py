@@ -191,3 +191,34 @@ def test_transpose_batched_validates_before_device_work():
         assert b"transpose item 1" in lib.vp_last_error()
     huge = (_TransposeItem * 1)(_TransposeItem(1, 1, 1 << 30, 1 << 20, 0, 0))
     assert lib.vp_transpose_batched(huge, p, 1, 0, None) == -2
+
+
+def test_launch_variant_rule_without_a_device():
+    """vp_launch_variant: the variant a single launch uses (256 compute units
+    assumed without a device).  A above 128 slices takes the strip width whose
+    grid fills the device best (rounds of workgroups x (P + 4)); below, the
+    H-based default, 2 slices per lane dropping to 1 on small grids; AT always
+    the H-based default."""
+    lib = voxelproj_amd.load()
+    lv = lib.vp_launch_variant
+    # (n_angles, SY, SX, H, PX) -> variant, worked out by hand for 256 CUs:
+    # 256^3: 12 x 16 x 1 = 192 blocks of 22 pixels (26) vs 16 x 16 = 256 of 16 (20)
+    assert lv(0, 192, 256, 256, 256, 256) == 416
+    # 288^3: 2 slice blocks; 16 strips of 18 pixels x 16 x 2 = 512 = two rounds (2 x 22)
+    assert lv(0, 192, 288, 288, 288, 288) == 418
+    # 320^3: 16 strips of 20 pixels x 32 = 512 (2 x 24) against 15 of 22 x 32 = 480 (2 x 26)
+    assert lv(0, 192, 320, 320, 320, 320) == 420
+    # 384^3: 24 strips of 16 x 32 = 768, three rounds (3 x 20) against 18 of 22 (3 x 26)
+    assert lv(0, 192, 384, 384, 384, 384) == 416
+    # 512^3: 24 strips of 22 x 32 = 768 (3 x 26) against 32 of 16 x 32 = 1024 (4 x 20)
+    assert lv(0, 192, 512, 512, 512, 512) == 4
+    assert lv(0, 1800, 1024, 1024, 512, 1024) == 4
+    # at most 128 slices: the H-based default, 2 -> 1 on grids smaller than the device
+    assert lv(0, 192, 128, 128, 128, 128) == 1
+    assert lv(0, 1800, 512, 512, 128, 512) == 2
+    assert lv(0, 360, 256, 256, 32, 256) == 1
+    for H, v in ((32, 32), (64, 64), (128, 128), (129, 4), (512, 4)):
+        assert lv(1, 192, 256, 256, H, 256) == v == lib.vp_default_variant(1, H)
+    assert lv(2, 192, 64, 64, 64, 64) == -1
+    assert lv(0, 0, 64, 64, 64, 64) == -1
+    assert lv(0, 10, 64, 64, -1, 64) == -1

@@ -7,8 +7,9 @@ these choices changes the order in which an output adds its terms, so every
 A configuration with 1 or 2 slices per lane must give the plain operator's
 result bit for bit.  The batched launches with an explicit variant use the
 default configurations (28-pixel strips), so comparing them with the single
-launch checks exactly that.  (4 slices per lane uses 22-pixel strips and is
-covered by the tolerance tests.)
+launch checks exactly that.  Above 128 slices a single launch of A takes
+the strip width (22, 20, 18 or 16 pixels) whose grid fills the device best,
+and equals the batched launch of that variant bit for bit.
 AT: 4 slices per lane, 256 slices per block above 128 slices, below that
 rows of 8, 16 or 32 lanes per voxel (blocks of 32, 64 or 128 slices); all of
 them add each voxel's terms in the same order, so the single launch equals
@@ -79,6 +80,45 @@ def test_small_configurations_match_oracle(shape):
     for got, ref in ((got_y, ref_y), (got_x, ref_x)):
         assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-6
         assert np.abs(got - ref).max() / np.abs(ref).max() <= 1e-5
+
+
+# Single launches of A above 128 slices pick the strip width (22, 20, 18 or
+# 16 pixels: variants 4, 420, 418, 416) whose grid fills the device best;
+# these sizes reach each of them on a 256-CU MI355X.
+STRIP_SHAPES = [(256, 256, 256, 192, 256), (288, 288, 288, 192, 288), (320, 320, 320, 192, 320),
+                (160, 200, 300, 96, 230), (384, 64, 260, 40, 390), (256, 256, 300, 720, 256)]
+
+
+def test_single_launch_strip_width_equals_its_batched_variant():
+    """A single launch equals the batched launch of the variant
+    vp_launch_variant names for its geometry bit for bit, and agrees with the
+    other strip widths to float32 rounding; a few slices against the FP64
+    oracle."""
+    from voxelproj_amd import _lib
+    lib = _lib.load()
+    seen = set()
+    for shape in STRIP_SHAPES:
+        SY, SX, H, A, PX = shape
+        print(f"strip width {shape}", flush=True)
+        x, _, ang = _inputs(shape, 41)
+        v = int(lib.vp_launch_variant(0, A, SY, SX, H, PX))
+        assert v in (4, 416, 418, 420), v
+        seen.add(v)
+        ya = ops.project(x, ang, PX)
+        yv = torch.empty_like(ya)
+        batch.BatchPlan("A", [x], [yv], [ang], accumulate=False, variant=v).launch()
+        assert torch.equal(yv, ya), (shape, v)
+        for w in (4, 416, 418, 420):
+            if w != v:
+                yw = torch.empty_like(ya)
+                batch.BatchPlan("A", [x], [yw], [ang], accumulate=False, variant=w).launch()
+                assert float((yw.double() - ya.double()).norm() / ya.double().norm()) <= 1e-6, (shape, w)
+        hs = slice(H - 3, H)
+        ref = vpo.forward_f64(x[:, :, hs].contiguous().cpu().numpy(), ang.cpu().numpy(), PX, 0)
+        got = ya[:, :, hs].cpu().numpy().astype(np.float64)
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-6, shape
+    if torch.cuda.get_device_properties(0).multi_processor_count == 256:
+        assert seen == {4, 416, 418, 420}, seen
 
 
 # Forward variant 32 (1 slice per lane, two angles per sweep wave, 32

@@ -17,9 +17,9 @@ EXPORTS = ("vp_forward", "vp_backward", "vp_forward_f64", "vp_backward_f64",
            "vp_forward_ws", "vp_backward_ws", "vp_workspace_elems", "vp_release_workspaces",
            "vp_project_noalloc", "vp_noalloc_scratch_elems",
            "vp_vshift", "vp_forward_batched", "vp_backward_batched", "vp_transpose_batched",
-           "vp_default_variant",
+           "vp_default_variant", "vp_launch_variant",
            "vp_batch_blocks", "vp_last_error", "vp_abi_version")
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 _lib = None
 _lock = threading.Lock()
@@ -95,6 +95,8 @@ def load():
         lib.vp_transpose_batched.restype = ctypes.c_int
         lib.vp_default_variant.argtypes = [ctypes.c_int, i64]
         lib.vp_default_variant.restype = ctypes.c_int
+        lib.vp_launch_variant.argtypes = [ctypes.c_int, ctypes.c_int, i64, i64, i64, i64]
+        lib.vp_launch_variant.restype = ctypes.c_int
         lib.vp_batch_blocks.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, i64, i64, i64, i64]
         lib.vp_batch_blocks.restype = i64
         lib.vp_last_error.argtypes = []

@@ -33,7 +33,7 @@ import torch
 
 from . import _lib, ops, recon
 
-FORWARD_VARIANTS = (1, 2, 4, 32)
+FORWARD_VARIANTS = (1, 2, 4, 416, 418, 420, 32)
 BACKWARD_VARIANTS = (1, 2, 4, 32, 64, 128)
 
 
@@ -178,7 +178,12 @@ class BatchPlan:
 
 
 def _default_variant(op, shapes):
+    """The default of the median H; for a plan of one instance, the variant
+    its single launch uses (the same bits)."""
     lib = _lib.load()
+    if len(shapes) == 1:
+        SY, SX, H, A_, PX = shapes[0]
+        return int(lib.vp_launch_variant(0 if op == "A" else 1, A_, SY, SX, H, PX))
     hs = sorted(s[2] for s in shapes)
     return int(lib.vp_default_variant(0 if op == "A" else 1, hs[len(hs) // 2]))
 
@@ -256,6 +261,9 @@ class GroupedPlan:
     for -- the one the single-instance operator uses, so each result equals
     the plain operator's bit for bit -- while each group still runs as one
     persistent launch.  ``variant_of(H)`` overrides the grouping rule.
+    Forward instances above 128 slices take the strip width their single
+    launch would (``vp_launch_variant``: 4, 416, 418 or 420, by how well the
+    instance's own grid fills the device), for the same bits.
     Forward instances with H <= 32 form a group of their own (variant 1 by
     default, like the single-instance operator), so that ``autotune_groups``
     can give them variant 32 (two angles per sweep wave: no idle lanes at
@@ -275,6 +283,10 @@ class GroupedPlan:
             # H: the last extent of volumes and sinograms in z_order 0, the first in 2
             H = int(t.shape[2] if z_order == 0 else t.shape[0])
             v = int(variant_of(H)) if variant_of is not None else int(lib.vp_default_variant(code, H))
+            if variant_of is None and op == "A" and v == 4:
+                SY, SX, _ = ops.vol_dims(t.shape, z_order)
+                A_, PX, _ = ops.sino_dims(outs[i].shape, z_order)
+                v = int(lib.vp_launch_variant(0, int(A_), int(SY), int(SX), H, int(PX)))
             narrow = variant_of is None and op == "A" and H <= 32
             groups.setdefault((v, narrow), []).append(i)
         self.op = op

@@ -1268,7 +1268,8 @@ int single_variant(int op, int A, int64_t SY, int64_t SX, int64_t H, int64_t PX)
     // AT's configurations (4 slices per lane, 256 slices per block or rows
     // of 8, 16, 32 lanes per voxel below 129 slices) all add their terms in
     // the same order, and the row configurations fill any volume's lanes
-    if (op == 1 || v != 2) return v;  // 4 slices per lane is kept: it adds terms in its own order
+    if (op == 0 && v == 4) return forward_strip_variant(A, H, PX);
+    if (op == 1 || v != 2) return v;
     const int cus = device_cus();
     if (op == 0) return forward_blocks(2, A, H, PX) < cus ? 1 : 2;
     // AT: with fewer 2-slice workgroups than CUs, 1 slice per lane.  Else

@@ -100,7 +100,7 @@ int validate_batch(int op, int variant, const vp_batch_item* items, int n)
 
 extern "C" {
 
-int vp_abi_version(void) { return 3; }
+int vp_abi_version(void) { return 4; }
 
 const char* vp_last_error(void) { return g_last_error.c_str(); }
 
@@ -559,6 +559,12 @@ int vp_vshift(int z_order, int transpose, const float* in, float* out, int n_ang
 }
 
 int vp_default_variant(int op, int64_t H) { return vp::default_variant(op, H); }
+
+int vp_launch_variant(int op, int n_angles, int64_t SY, int64_t SX, int64_t H, int64_t PX)
+{
+    if ((op != 0 && op != 1) || n_angles <= 0 || SY <= 0 || SX <= 0 || H <= 0 || PX <= 0) return -1;
+    return vp::single_variant(op, n_angles, SY, SX, H, PX);
+}
 
 int64_t vp_batch_blocks(int op, int variant, int n_angles, int64_t SY, int64_t SX, int64_t H,
                         int64_t PX)

@@ -886,6 +886,14 @@ template <> struct FwdCfg<2> { static constexpr int HL = 2, AW = 1, P = 28, NW =
 // the sweep).  Pieces of up to 69 voxels: 140 KB of staging, 18 KB of slot
 // tables.
 template <> struct FwdCfg<4> { static constexpr int HL = 4, AW = 1, P = 22, NW = 12, NS = 4, UMAX = 69, WPE = 4; };
+// The same with 16-, 18- and 20-pixel strips (variants 416, 418, 420): more,
+// shorter workgroups, for grids that 22-pixel strips would leave a CU round
+// partly empty (forward_strip_variant).  Another strip width stages another
+// union of segments per line, so where a line is split into pieces the
+// pieces end elsewhere: float32 rounding against variant 4.
+template <> struct FwdCfg<416> { static constexpr int HL = 4, AW = 1, P = 16, NW = 12, NS = 4, UMAX = 69, WPE = 4; };
+template <> struct FwdCfg<418> { static constexpr int HL = 4, AW = 1, P = 18, NW = 12, NS = 4, UMAX = 69, WPE = 4; };
+template <> struct FwdCfg<420> { static constexpr int HL = 4, AW = 1, P = 20, NW = 12, NS = 4, UMAX = 69, WPE = 4; };
 // 32 slices per block (variant 32, for H <= 32): 1 slice per lane and two
 // angles per sweep wave, 24 angles per block, otherwise as variant 1.  Where
 // no line's staged union exceeds UMAX (volumes of at most 128 voxels per
@@ -965,6 +973,9 @@ hipError_t launch_forward_z0(const float* x, float* y, const float* angles, cons
                 return forward_single<FwdSmall>(a, s);
             return forward_single<FwdCfg<1>>(a, s);
         case 2: return forward_single<FwdCfg<2>>(a, s);
+        case 416: return forward_single<FwdCfg<416>>(a, s);
+        case 418: return forward_single<FwdCfg<418>>(a, s);
+        case 420: return forward_single<FwdCfg<420>>(a, s);
         default: return forward_single<FwdCfg<4>>(a, s);
     }
 }
@@ -976,9 +987,34 @@ hipError_t launch_forward_batched(int variant, const BatchItem* items_host,
         case 1: return forward_batched<FwdCfg<1>>(items_host, items_dev, n, grid, s);
         case 2: return forward_batched<FwdCfg<2>>(items_host, items_dev, n, grid, s);
         case 4: return forward_batched<FwdCfg<4>>(items_host, items_dev, n, grid, s);
+        case 416: return forward_batched<FwdCfg<416>>(items_host, items_dev, n, grid, s);
+        case 418: return forward_batched<FwdCfg<418>>(items_host, items_dev, n, grid, s);
+        case 420: return forward_batched<FwdCfg<420>>(items_host, items_dev, n, grid, s);
         case 32: return forward_batched<FwdCfg<32>>(items_host, items_dev, n, grid, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+int forward_strip_variant(int A, int64_t H, int64_t PX)
+{
+    // Measured at 192 angles (one MI355X, 256 CUs): 256^3 22 -> 16 pixels
+    // 0.75 -> 0.53 ms, 288^3 22 -> 18 1.40 -> 1.25 ms, 320^3 22 -> 20
+    // 1.68 -> 1.64 ms, 384^3 22 -> 16 2.48 -> 2.13 ms, 448^3 22 -> 20
+    // 3.30 -> 3.14 ms; 22 pixels kept at 512^3, 512^2 x 256 x 720 and
+    // 1024^2 x 512 x 1800, as the rule picks.
+    const int64_t cus = device_cus();
+    int best = 4;
+    int64_t best_cost = -1;
+    for (const int v : {4, 420, 418, 416}) {  // ties keep the wider strip
+        const int64_t nb = forward_blocks(v, A, H, PX);
+        const int P = v == 4 ? 22 : v - 400;
+        const int64_t cost = (nb + cus - 1) / cus * (P + 4);
+        if (best_cost < 0 || cost < best_cost) {
+            best = v;
+            best_cost = cost;
+        }
+    }
+    return best;
 }
 
 int64_t forward_blocks(int variant, int A, int64_t H, int64_t PX)
@@ -988,6 +1024,9 @@ int64_t forward_blocks(int variant, int A, int64_t H, int64_t PX)
         case 1: forward_grid<FwdCfg<1>::P, FwdCfg<1>::NW, 1>(A, H, PX, nbp, nba, nbh); break;
         case 2: forward_grid<FwdCfg<2>::P, FwdCfg<2>::NW, 2>(A, H, PX, nbp, nba, nbh); break;
         case 4: forward_grid<FwdCfg<4>::P, FwdCfg<4>::NW, 4>(A, H, PX, nbp, nba, nbh); break;
+        case 416: forward_grid<FwdCfg<416>::P, FwdCfg<416>::NW, 4>(A, H, PX, nbp, nba, nbh); break;
+        case 418: forward_grid<FwdCfg<418>::P, FwdCfg<418>::NW, 4>(A, H, PX, nbp, nba, nbh); break;
+        case 420: forward_grid<FwdCfg<420>::P, FwdCfg<420>::NW, 4>(A, H, PX, nbp, nba, nbh); break;
         case 32: forward_grid<FwdCfg<32>::P, FwdCfg<32>::NW, 1, 2>(A, H, PX, nbp, nba, nbh); break;
         default: return -1;
     }

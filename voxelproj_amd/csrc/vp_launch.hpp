@@ -37,14 +37,21 @@ struct BatchItem {
 // Kernel variants (the autotuner's search space): 1, 2, 4 = slices per lane
 // (blocks of 64, 128, 256 slices); for AT also 32, 64, 128 = blocks of that
 // many slices at 4 slices per lane, a wave's lanes split into rows of 8, 16,
-// 32 lanes that hold one voxel each.
-constexpr int kForwardVariants[] = {1, 2, 4};
+// 32 lanes that hold one voxel each; for A also 416, 418, 420 = 4 slices per
+// lane with 16-, 18- and 20-pixel strips (4 itself: 22 pixels), and 32 = 1
+// slice per lane with two angles per sweep wave.
+constexpr int kForwardVariants[] = {1, 2, 4, 416, 418, 420, 32};
 constexpr int kBackwardVariants[] = {1, 2, 4, 32, 64, 128};
 int default_variant(int op, int64_t H);  // op 0 = A, 1 = AT
 // Variant of a single (non-batched) launch: the H-based default, except that
 // 2 slices per lane drops to 1 for A when its grid has fewer workgroups than
-// the device has CUs.
+// the device has CUs, and that A's 4 slices per lane takes the strip width
+// whose grid fills the device best (forward_strip_variant).
 int single_variant(int op, int A, int64_t SY, int64_t SX, int64_t H, int64_t PX);
+// A at 4 slices per lane: 4, 416, 418 or 420, the strip width P that
+// minimises rounds of resident workgroups x (P + 4), the per-workgroup cost
+// measured to grow about as P + 4 (one workgroup per CU).
+int forward_strip_variant(int A, int64_t H, int64_t PX);
 int device_cus();  // compute units of the current device (cached per device)
 // Work items (blocks) of one instance under a variant; -1 for an unknown variant.
 int64_t forward_blocks(int variant, int A, int64_t H, int64_t PX);

@@ -128,8 +128,10 @@ def _safe_reciprocal_(t, eps=1e-6):
     return t
 
 
-def _variant(op, H):
-    return int(_lib.load().vp_default_variant(0 if op == "A" else 1, H))
+def _variant(op, r):
+    """The variant a single launch of `op` on request r uses (a batch grouped
+    by it gives the single-launch results bit for bit)."""
+    return int(_lib.load().vp_launch_variant(0 if op == "A" else 1, r.n_angles, r.SY, r.SX, r.H, r.PX))
 
 
 def _run_chain(r):
@@ -186,7 +188,7 @@ class Server:
 
     def _blocks(self, op, r):
         code = 0 if op == "A" else 1
-        return _lib.load().vp_batch_blocks(code, _variant(op, r.H), r.n_angles, r.SY, r.SX,
+        return _lib.load().vp_batch_blocks(code, _variant(op, r), r.n_angles, r.SY, r.SX,
                                            r.H, r.PX)
 
     def fills_device(self, req):
@@ -295,7 +297,7 @@ class Server:
             if len(group) > 1:
                 by_variant = {}
                 for r, (_, inp, out, _post) in zip(group, steps):
-                    by_variant.setdefault((_variant(op, r.H), r.z_order), []).append((inp, out, r.angles))
+                    by_variant.setdefault((_variant(op, r), r.z_order), []).append((inp, out, r.angles))
                 for (v, z), jobs in sorted(by_variant.items()):
                     self._plan(op, v, z, jobs).launch()
             else:
