@@ -50,9 +50,10 @@ struct Task {
 // and read by all NW sweep waves, so each volume load feeds up to NW angles.
 // Lines are split into pieces of at most UMAX voxels.  The work of a piece is
 // split by role, one barrier per piece:
-//   staging waves: find the next piece, issue its global->LDS DMA into the
-//     other buffer, and build the footprint slot table of each of their AS
-//     angles for it in LDS;
+//   staging waves: issue the piece's global->LDS DMA into the other buffer,
+//     build the footprint slot tables of their angles for it in LDS, and
+//     find the piece after it (at 4 slices per lane one staging wave finds
+//     the pieces for all and builds no tables);
 //   sweep waves: read their angle's slot table for the current piece and
 //     sweep it (the FMAs).
 // So the latency-bound part of a piece (piece iteration, footprints, slot
