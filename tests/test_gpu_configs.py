@@ -187,3 +187,29 @@ def test_blocks_with_both_line_directions(shape):
         tail = slice(H - 3, H)
         ref_t = vpo.forward_f64(x[:, :, tail].contiguous().cpu().numpy(), MIXED, PX, 0)
         assert np.linalg.norm(got[:, :, tail] - ref_t) / np.linalg.norm(ref_t) <= 1e-6, k
+
+
+# H above 256 with a last block of at most 128 slices: single launches of AT
+# give that block to a row kernel of its size.
+RAGGED_AT_SHAPES = [(40, 36, 257, 45, 52), (33, 41, 288, 29, 50), (24, 30, 300, 60, 40),
+                    (20, 20, 320, 17, 31), (16, 24, 384, 23, 26), (9, 13, 600, 11, 20), (1, 1, 259, 5, 3)]
+
+
+@pytest.mark.parametrize("shape", RAGGED_AT_SHAPES)
+def test_at_ragged_tail_equals_one_kernel(shape):
+    """The split launch (4-slice kernel on whole 256-slice blocks, row kernel
+    on the rest) equals the 4-slice kernel over all of H bit for bit, with
+    accumulation, and the FP64 oracle on the last slices."""
+    SY, SX, H, A, PX = shape
+    print(f"ragged AT {shape}", flush=True)
+    _, y, ang = _inputs(shape, 53)
+    x0 = torch.randn((SY, SX, H), device=DEV, generator=torch.Generator(device=DEV).manual_seed(H))
+    got = ops.backproject(y, ang, SY, SX, 0, out=x0.clone())
+    one = x0.clone()
+    batch.BatchPlan("AT", [y], [one], [ang], accumulate=True, variant=4).launch()
+    assert torch.equal(got, one)
+    hs = slice(H - 4, H)
+    ref = vpo.backward_f64(y[:, :, hs].contiguous().cpu().numpy().astype(np.float64), ang.cpu().numpy(), SY, SX, 0)
+    sums = (got - x0)[:, :, hs].cpu().numpy().astype(np.float64)
+    scale = np.linalg.norm(ref)
+    assert np.linalg.norm(sums - ref) / scale <= 1e-6 + 2e-7 * np.linalg.norm(x0[:, :, hs].cpu().numpy()) / scale

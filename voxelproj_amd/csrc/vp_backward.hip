@@ -392,7 +392,7 @@ __device__ __forceinline__ void backproject4_block(const OpArgs& g, const int lo
     const int A = g.A, SY = g.SY, SX = g.SX, H = g.H, PX = g.PX;
     const double scale = g.scale, inv_scale = g.inv_scale;
     int nbx, nby, nbh_;
-    backward4_grid<TX, TY, NW, AC, WIN>(SY, SX, H, nbx, nby, nbh_);
+    backward4_grid<TX, TY, NW, AC, WIN>(SY, SX, g.hn > 0 ? g.hn : H - g.h0, nbx, nby, nbh_);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -401,7 +401,7 @@ __device__ __forceinline__ void backproject4_block(const OpArgs& g, const int lo
     const int by = (logical / nbx) % nby;
     const int bhc = logical / (nbx * nby);
     const int tx0 = bx * TX, ty0 = by * TY;
-    const int hb = bhc * ROW;
+    const int hb = g.h0 + bhc * ROW;
     const float xc = 0.5f * (float)(SX - 1);
     const float yc = 0.5f * (float)(SY - 1);
     const bool vec4 = (H & 3) == 0;
@@ -752,7 +752,7 @@ __device__ __forceinline__ void backproject_rows_block(const OpArgs& g, const in
     const int A = g.A, SY = g.SY, SX = g.SX, H = g.H, PX = g.PX;
     const double scale = g.scale, inv_scale = g.inv_scale;
     int nbx, nby, nbh_;
-    backward_rows_grid<TX, TY, NW, AC, WIN, RW>(SY, SX, H, nbx, nby, nbh_);
+    backward_rows_grid<TX, TY, NW, AC, WIN, RW>(SY, SX, g.hn > 0 ? g.hn : H - g.h0, nbx, nby, nbh_);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -761,7 +761,7 @@ __device__ __forceinline__ void backproject_rows_block(const OpArgs& g, const in
     const int by = (logical / nbx) % nby;
     const int bhc = logical / (nbx * nby);
     const int tx0 = bx * TX, ty0 = by * TY;
-    const int hb = bhc * ROW;
+    const int hb = g.h0 + bhc * ROW;
     const float xc = 0.5f * (float)(SX - 1);
     const float yc = 0.5f * (float)(SY - 1);
     const bool vec4 = (H & 3) == 0;
@@ -995,6 +995,8 @@ __device__ __forceinline__ OpArgs item_args(const BatchItem& it)
     g.PX = it.PX;
     g.accumulate = it.accumulate;
     g.out_z2 = 0;
+    g.h0 = 0;
+    g.hn = 0;
     const int bits = hi_bits(it.SY, it.SX, it.PX);
     g.du = nullptr;
     g.scale = ldexp(1.0, bits);
@@ -1131,7 +1133,8 @@ hipError_t backward_rows_single(const OpArgs& g, hipStream_t s)
 {
     using C = BwdRowsCfg<RW>;
     int nbx, nby, nbh;
-    backward_rows_grid<C::TX, C::TY, C::NW, C::AC, C::WIN, RW>(g.SY, g.SX, g.H, nbx, nby, nbh);
+    backward_rows_grid<C::TX, C::TY, C::NW, C::AC, C::WIN, RW>(g.SY, g.SX, g.hn > 0 ? g.hn : g.H - g.h0, nbx, nby,
+                                                               nbh);
     const int64_t nb = (int64_t)nbx * nby * nbh;
     if (nb > 0x7fffffff) return hipErrorInvalidValue;
     hipLaunchKernelGGL((backproject_rows_z0_kernel<C::TX, C::TY, C::NW, C::AC, C::WIN, C::TAB, RW>),
@@ -1160,7 +1163,7 @@ hipError_t backward4_single(const OpArgs& g, hipStream_t s)
 {
     using C = Bwd4Cfg;
     int nbx, nby, nbh;
-    backward4_grid<C::TX, C::TY, C::NW, C::AC, C::WIN>(g.SY, g.SX, g.H, nbx, nby, nbh);
+    backward4_grid<C::TX, C::TY, C::NW, C::AC, C::WIN>(g.SY, g.SX, g.hn > 0 ? g.hn : g.H - g.h0, nbx, nby, nbh);
     const int64_t nb = (int64_t)nbx * nby * nbh;
     if (nb > 0x7fffffff) return hipErrorInvalidValue;
     hipLaunchKernelGGL((backproject4_z0_kernel<C::TX, C::TY, C::NW, C::AC, C::WIN, C::TAB>),
@@ -1302,6 +1305,8 @@ OpArgs backward_args(const float* y, float* x, const float* angles, const Geom& 
     a.PX = (int)g.PX;
     a.accumulate = accumulate;
     a.out_z2 = out_z2;
+    a.h0 = 0;
+    a.hn = 0;
     a.scale = ldexp(1.0, g.bits);
     a.inv_scale = ldexp(1.0, -g.bits);
     a.du = g.du;
@@ -1324,7 +1329,26 @@ hipError_t launch_backward_z0(const float* y, float* x, const float* angles, con
 {
     const OpArgs a = backward_args(y, x, angles, g, accumulate, 0);
     const int v = single_variant(1, g.A, g.SY, g.SX, g.H, g.PX);
-    if (v == 4) return backward4_single(a, s);
+    if (v == 4) {
+        // A ragged last block of at most 128 slices goes to a row kernel of
+        // its size instead of a 256-slice block with most lanes idle (the
+        // same terms in the same order, so the same bits).  The row kernel
+        // addresses an angle's sinogram rows through 32-bit byte offsets:
+        // PX * H * 4 must stay below 2^31.
+        const int tail = (int)(g.H % 256);
+        if (g.H > 256 && tail > 0 && tail <= 128 && g.PX * g.H < ((int64_t)1 << 29)) {
+            OpArgs main = a, rest = a;
+            main.hn = (int)g.H - tail;
+            rest.h0 = (int)g.H - tail;
+            rest.hn = tail;
+            const hipError_t e = backward4_single(main, s);
+            if (e != hipSuccess) return e;
+            if (tail <= 32) return backward_rows_single<8>(rest, s);
+            if (tail <= 64) return backward_rows_single<16>(rest, s);
+            return backward_rows_single<32>(rest, s);
+        }
+        return backward4_single(a, s);
+    }
     if (v == 32) return backward_rows_single<8>(a, s);
     if (v == 64) return backward_rows_single<16>(a, s);
     if (v == 128) return backward_rows_single<32>(a, s);

@@ -439,15 +439,25 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
                             if constexpr (AW > 1) r1 = rtab[SLOTS + q];
                             const float wl2 = AW > 1 ? (float)r1.x : rdlane(t1.x, q);
                             const float wr2 = AW > 1 ? (float)r1.y : rdlane(t1.y, q);
+                            // With two angles per wave a lane whose own angle has
+                            // no second voxel here runs this branch for the other
+                            // half: its staged row one voxel up may lie past the
+                            // piece (stale LDS), so it adds zeros instead of
+                            // zero-weighted products of that row.
+                            auto xb2 = xb;
+                            if constexpr (AW > 1) {
+#pragma unroll
+                                for (int j = 0; j < HL; ++j) xb2.a[j] = wl2 != 0.f ? xb.a[j] : 0.f;
+                            }
 #pragma unroll
                             for (int j = 0; j < HL; ++j) {
-                                if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wl2, xb.a[j], part[q - 2][j]);
-                                if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(wr2, xb.a[j], part[q - 1][j]);
+                                if (q - 2 >= 0 && q - 2 < P) part[q - 2][j] = fmaf(wl2, xb2.a[j], part[q - 2][j]);
+                                if (q - 1 >= 0 && q - 1 < P) part[q - 1][j] = fmaf(wr2, xb2.a[j], part[q - 1][j]);
                             }
                             if (q < P && ((has3b >> q) & 1u)) {
                                 const float w3 = AW > 1 ? (float)r1.z : rdlane(t1.z, q);
 #pragma unroll
-                                for (int j = 0; j < HL; ++j) part[q][j] = fmaf(w3, xb.a[j], part[q][j]);
+                                for (int j = 0; j < HL; ++j) part[q][j] = fmaf(w3, xb2.a[j], part[q][j]);
                             }
                         }
                     }
@@ -736,6 +746,8 @@ __device__ __forceinline__ OpArgs item_args(const BatchItem& it)
     g.PX = it.PX;
     g.accumulate = it.accumulate;
     g.out_z2 = 0;
+    g.h0 = 0;
+    g.hn = 0;
     const int bits = hi_bits(it.SY, it.SX, it.PX);
     g.du = nullptr;
     g.scale = ldexp(1.0, bits);
@@ -965,6 +977,8 @@ hipError_t launch_forward_z0(const float* x, float* y, const float* angles, cons
     a.PX = (int)g.PX;
     a.accumulate = accumulate;
     a.out_z2 = 0;
+    a.h0 = 0;
+    a.hn = 0;
     a.scale = ldexp(1.0, g.bits);
     a.inv_scale = ldexp(1.0, -g.bits);
     a.du = g.du;

@@ -23,6 +23,11 @@ struct OpArgs {
     double scale, inv_scale;  // 2^B and 2^-B of the cos/sin hi split
     const float* du;          // per-angle detector translation (pixels), or null
     int out_z2;               // AT, 4 slices per lane: out is (H, SY, SX) instead of (SY, SX, H)
+    // Slices h0 .. h0 + hn - 1 of the H (hn = 0: h0 .. H - 1) are the ones
+    // this launch covers; strides and bounds still use H.  Honoured by AT's
+    // 4-slice and row kernels (single launches split a ragged H between
+    // them); every other launch passes 0, 0.
+    int h0, hn;
 };
 
 // One instance of a batched launch (layout shared with the C ABI's vp_batch_item).
