@@ -213,3 +213,33 @@ def test_at_ragged_tail_equals_one_kernel(shape):
     sums = (got - x0)[:, :, hs].cpu().numpy().astype(np.float64)
     scale = np.linalg.norm(ref)
     assert np.linalg.norm(sums - ref) / scale <= 1e-6 + 2e-7 * np.linalg.norm(x0[:, :, hs].cpu().numpy()) / scale
+
+
+# H above 256 with a last block of at most 128 slices: single launches of A
+# give that block to a 1- or 2-slice kernel of the same strips and pieces.
+RAGGED_A_SHAPES = [(40, 36, 257, 45, 52), (33, 41, 288, 29, 50), (24, 30, 300, 60, 40),
+                   (20, 20, 320, 17, 31), (16, 24, 384, 23, 26), (9, 13, 600, 11, 20), (1, 1, 259, 5, 3),
+                   (256, 256, 288, 192, 256)]
+
+
+@pytest.mark.parametrize("shape", RAGGED_A_SHAPES)
+def test_a_ragged_tail_equals_one_kernel(shape):
+    """The split launch (4-slice kernel on whole 256-slice blocks, a 1- or
+    2-slice kernel of the same strips on the rest) equals the 4-slice kernel
+    of the same strip width over all of H bit for bit, with accumulation, and
+    the FP64 oracle on the last slices."""
+    from voxelproj_amd import _lib
+    SY, SX, H, A, PX = shape
+    print(f"ragged A {shape}", flush=True)
+    x, _, ang = _inputs(shape, 59)
+    y0 = torch.randn((A, PX, H), device=DEV, generator=torch.Generator(device=DEV).manual_seed(H))
+    got = ops.project(x, ang, PX, 0, out=y0.clone())
+    v = int(_lib.load().vp_launch_variant(0, A, SY, SX, H, PX))
+    one = y0.clone()
+    batch.BatchPlan("A", [x], [one], [ang], accumulate=True, variant=v).launch()
+    assert torch.equal(got, one), v
+    hs = slice(H - 4, H)
+    ref = vpo.forward_f64(x[:, :, hs].contiguous().cpu().numpy(), ang.cpu().numpy(), PX, 0)
+    sums = (got - y0)[:, :, hs].cpu().numpy().astype(np.float64)
+    scale = np.linalg.norm(ref)
+    assert np.linalg.norm(sums - ref) / scale <= 1e-6 + 2e-7 * np.linalg.norm(y0[:, :, hs].cpu().numpy()) / scale

@@ -92,7 +92,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     const double scale = g.scale, inv_scale = g.inv_scale;
     const int accumulate = g.accumulate;
     int nbp, nba, nbh;
-    forward_grid<P, NW, HL, AW>(A, H, PX, nbp, nba, nbh);
+    forward_grid<P, NW, HL, AW>(A, g.hn > 0 ? g.hn : H - g.h0, PX, nbp, nba, nbh);
     static_assert((P + 5) * 1.4143 + 4 <= kSeg, "segment may exceed one wave");
     static_assert(AW == 1 || (AW == 2 && HL == 1), "two angles per wave only at 1 slice per lane");
     constexpr int NA = NW * AW;                   // angles per block
@@ -100,10 +100,13 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     constexpr int SLOTS = P + 2;                 // first pixels p0-2 .. p0+P-1
     static_assert(SLOTS <= 32, "slot masks are 32-bit");
     constexpr int VOX = 64 * HL / AW;            // floats per staged voxel (slices per block)
-    // floats per buffer: UMAX voxels plus one, so that the sweep's read of a
-    // slot's second voxel (one voxel above the first) stays inside the array
-    // even where the slot has none
-    constexpr int STAGE = (UMAX + 1) * VOX;
+    // floats per buffer: UMAX voxels rounded up to whole DMA groups (the last
+    // DMA instruction of a piece fills a whole group), plus one, so that the
+    // sweep's read of a slot's second voxel (one voxel above the first) stays
+    // inside the array even where the slot has none
+    constexpr int GRP = 1024 / (VOX * 4);  // voxels per 16-byte DMA (a multiple of the dword DMA's)
+    constexpr int UCAP = (UMAX + GRP - 1) / GRP * GRP;
+    constexpr int STAGE = (UCAP + 1) * VOX;
     // Staging double buffer as two separate arrays: every access names its
     // buffer statically, so the compiler can tell that the sweep's LDS reads
     // of one buffer do not alias the DMA into the other and does not wait for
@@ -130,7 +133,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     const int bp = (logical / nba) % nbp;
     const int bh = logical / (nba * nbp);
 
-    const int hb = bh * VOX;                     // first slice of the block
+    const int hb = g.h0 + bh * VOX;              // first slice of the block
     const int p0 = bp * P;
     const int pl = p0 - 1;
     const int ph = p0 + P;
@@ -158,7 +161,7 @@ __device__ __forceinline__ void forward_block(const OpArgs& g, const int logical
     constexpr int RPI = 1024 / (VOX * 4);    // voxels per 16-byte-per-lane DMA
     constexpr int LPR = 64 / RPI;            // lanes per voxel
     constexpr int DPI = VOX >= 64 ? 1 : 64 / VOX;  // voxels per dword DMA
-    static_assert(UMAX % RPI == 0 && UMAX % DPI == 0, "staging buffer must hold whole DMA groups");
+    static_assert(UCAP % RPI == 0 && UCAP % DPI == 0, "staging buffer must hold whole DMA groups");
     struct Dma {
         const float* base;
         int64_t st_along;
@@ -918,6 +921,12 @@ template <> struct FwdCfg<420> { static constexpr int HL = 4, AW = 1, P = 20, NW
 // so it pays on large grids only: batched launches offer it to the
 // autotuner; single launches keep variant 1.
 template <> struct FwdCfg<32> { static constexpr int HL = 1, AW = 2, P = 28, NW = 12, NS = 4, UMAX = 128, WPE = 4; };
+// The last block of a ragged H (at most 128 slices past whole 256-slice
+// blocks) in a single launch of a 4-slice variant: 1 or 2 slices per lane
+// with that variant's strips, angles and pieces (P, NW, UMAX = 69), so the
+// same terms in the same order as the 4-slice kernel over all of H.
+template <int TL, int TP>
+struct FwdTail { static constexpr int HL = TL, AW = 1, P = TP, NW = 12, NS = 4, UMAX = 69, WPE = 4; };
 
 // Narrower strips for 1 slice per lane, used by single launches of volumes
 // of at most UMAX = 128 voxels per side when the default grid would have fewer
@@ -936,7 +945,7 @@ template <typename C>
 hipError_t forward_single(const OpArgs& g, hipStream_t s)
 {
     int nbp, nba, nbh;
-    forward_grid<C::P, C::NW, C::HL, C::AW>(g.A, g.H, g.PX, nbp, nba, nbh);
+    forward_grid<C::P, C::NW, C::HL, C::AW>(g.A, g.hn > 0 ? g.hn : g.H - g.h0, g.PX, nbp, nba, nbh);
     const int64_t nb = (int64_t)nbp * nba * nbh;
     if (nb > 0x7fffffff) return hipErrorInvalidValue;
     hipLaunchKernelGGL((forward_z0_kernel<C::P, C::NW, C::NS, C::HL, C::UMAX, C::WPE, C::AW>), dim3((unsigned)nb),
@@ -988,10 +997,32 @@ hipError_t launch_forward_z0(const float* x, float* y, const float* angles, cons
                 return forward_single<FwdSmall>(a, s);
             return forward_single<FwdCfg<1>>(a, s);
         case 2: return forward_single<FwdCfg<2>>(a, s);
-        case 416: return forward_single<FwdCfg<416>>(a, s);
-        case 418: return forward_single<FwdCfg<418>>(a, s);
-        case 420: return forward_single<FwdCfg<420>>(a, s);
-        default: return forward_single<FwdCfg<4>>(a, s);
+        default: break;
+    }
+    // 4 slices per lane.  A ragged last block of at most 128 slices goes to
+    // a 1- or 2-slice kernel of the same strips and pieces instead of a
+    // 256-slice block with most lanes idle (the same bits).  Its DMA groups
+    // several voxels per instruction through 32-bit lane offsets: at most
+    // 3 * SX * H + H floats.
+    const int v = single_variant(0, g.A, g.SY, g.SX, g.H, g.PX);
+    const int tail = (int)(g.H % 256);
+    const bool split = g.H > 256 && tail > 0 && tail <= 128 && 3 * g.SX * g.H + g.H < ((int64_t)1 << 29);
+    OpArgs main = a, rest = a;
+    if (split) {
+        main.hn = (int)g.H - tail;
+        rest.h0 = (int)g.H - tail;
+        rest.hn = tail;
+    }
+    auto both = [&](auto cfg4, auto tail1, auto tail2) {
+        const hipError_t e = forward_single<decltype(cfg4)>(main, s);
+        if (e != hipSuccess || !split) return e;
+        return tail <= 64 ? forward_single<decltype(tail1)>(rest, s) : forward_single<decltype(tail2)>(rest, s);
+    };
+    switch (v) {
+        case 416: return both(FwdCfg<416>{}, FwdTail<1, 16>{}, FwdTail<2, 16>{});
+        case 418: return both(FwdCfg<418>{}, FwdTail<1, 18>{}, FwdTail<2, 18>{});
+        case 420: return both(FwdCfg<420>{}, FwdTail<1, 20>{}, FwdTail<2, 20>{});
+        default: return both(FwdCfg<4>{}, FwdTail<1, 22>{}, FwdTail<2, 22>{});
     }
 }
 
